@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark: Metropolis site-updates/s of the tile-sweep kernel on one MI355X.
+
+Workload (one "step" = one sweep = N attempted single-site updates):
+  128^3 periodic lattice, DipoleCutOff = 3 (122 neighbours), CageStrain = 1,
+  Efield = 0, K = 0, T = 300 K, FP64 unit dipoles uniform on the sphere
+  (synthetic, drawn on the device from a fixed seed).
+Also reported: the batched form (96 replicas of 32^3 over 32 temperatures x 3
+fields plus 32 replicas of uneven sizes 16^3..48^3, one launch per coarse
+colour for all of them) against a single 32^3 replica, and the CPU baseline
+(the float-faithful C restatement of the reference hot path, one process per
+core, as the reference is run).
+
+Prints ONE JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= dense FP64 matrix) peak
+PEAK_HBM_GBS = 8000.0
+
+
+def flops_per_update(n_nb: int, n_nn: int) -> int:
+    """Algorithmic FP64 flops of one attempted update: 19 per neighbour,
+    3 per nearest neighbour, 40 per site (proposal, dE, acceptance)."""
+    return 19 * n_nb + 3 * n_nn + 40
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds_target: float = 10.0, procs: int | None = None):
+    """Time oracle/sn_cpu_bench (float-faithful port of MC_moves, gcc -O3,
+    one thread per process) on the 128^3 rc=3 workload: one process alone,
+    then `procs` processes at once (distinct T, as the reference's GNU-parallel
+    runs).  Sample: a bounded number of moves per process."""
+    exe = os.path.join(ROOT, "oracle", "sn_cpu_bench")
+    if not os.path.exists(exe):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"])
+    # calibrate: 0.5 M moves
+    out = json.loads(subprocess.check_output([exe, "128", "128", "128", "3", "300", "500000"]))
+    rate1 = out["rate"]
+    moves = int(max(1e6, rate1 * seconds_target))
+    single = json.loads(subprocess.check_output([exe, "128", "128", "128", "3", "300", str(moves)]))
+    if procs is None:
+        try:
+            procs = len(os.sched_getaffinity(0))
+        except AttributeError:
+            procs = os.cpu_count() or 1
+        procs = max(1, min(procs, 16))
+    ps = [subprocess.Popen([exe, "128", "128", "128", "3", str(300 + 10 * i), str(moves)],
+                           stdout=subprocess.PIPE) for i in range(procs)]
+    rates = [json.loads(p.communicate()[0])["rate"] for p in ps]
+    return {
+        "value": sum(rates), "unit": "site-updates/s", "cores": procs, "kind": "port",
+        "single_core": single["rate"],
+        "sample": f"128^3 rc=3 T=300 random lattice, {moves} moves per process "
+                  f"(~{moves / single['rate']:.0f} s), float-faithful restatement of "
+                  f"MC_moves/site_energy with MT19937, gcc -O3, {procs} processes",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--L", type=int, default=128)
+    ap.add_argument("--rc", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the headline sweeps (for profiler runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(replicas and slabs would shard across GPUs, see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import starrynight_amd as sn
+
+    L, rc = args.L, args.rc
+    rp = sn.ReplicaParams(L, L, L, DipoleCutOff=rc, CageStrain=1.0, T=300, seed=0x5EED)
+    N = rp.N
+    n_nb = len(sn.gen_neighbour(rc, L))
+    F = flops_per_update(n_nb, 6)
+    ctx = sn.StarryNight(rp)
+    ctx.init_lattice_random(0)
+    cfg = ctx.sweep_config()
+    log(f"[bench] {L}^3 rc={rc} N={N} neighbours={n_nb} F={F} sweep config {cfg}")
+    ctx.MC_sweeps(args.warmup)
+    ctx.reset_counters()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.MC_sweeps(args.steps, sync=False)
+    ctx.sync()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    rate = args.steps * N / dt
+    cnt = ctx.counters()
+    log(f"[bench] {args.steps} sweeps in {dt * 1e3:.2f} ms -> {rate:.4e} site-updates/s, "
+        f"acceptance {cnt['accept'] / max(1, cnt['accept'] + cnt['reject']):.3f}")
+
+    # dominant kernel: per-launch HIP events on the library stream
+    ksweeps = max(2, min(args.steps, 20))
+    mean_ms, nl = ctx.time_sweep_kernel(ksweeps)
+    updates_per_launch = N * ksweeps / nl
+    achieved = updates_per_launch * F / (mean_ms * 1e-3) / 1e12
+    roof = {"bound": "fp64", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+            "kernel": "tile_sweep_kernel", "mean_launch_ms": round(mean_ms, 5),
+            "updates_per_launch": updates_per_launch, "flop_per_update": F}
+    log(f"[bench] tile_sweep_kernel mean {mean_ms * 1e3:.1f} us/launch over {nl} launches -> "
+        f"{achieved:.2f} TFLOP/s ({achieved / PEAK_FP64_TFLOPS:.1%} of FP64 peak)")
+    ctx.close()
+    if args.kernel_only:
+        print(json.dumps({"metric": "site-updates/s", "value": rate, "roofline": roof}))
+        return
+
+    batched = None
+    if not args.no_batched:
+        batched = run_batched(sn, rc, log)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        cpu = cpu_baseline()
+        log(f"[bench] CPU: 1 core {cpu['single_core']:.3e}/s, {cpu['cores']} cores {cpu['value']:.3e}/s")
+
+    line = {
+        "metric": "site-updates/s and FP64-roofline fraction at 128^3 rc=3; batched vs single replica",
+        "value": rate, "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{L}^3 periodic lattice, DipoleCutOff={rc}, single replica, one sweep per step",
+                   "X": L, "Y": L, "Z": L, "DipoleCutOff": rc, "CageStrain": 1.0, "Efield": [0, 0, 0],
+                   "T": 300, "neighbours": n_nb, "tiles": cfg["tiles"],
+                   "launches_per_sweep": cfg["launches_per_sweep"], "lanes_per_site": cfg["lanes_per_site"],
+                   "fine_colours": cfg["fine_colours"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu_all_cores": (rate / cpu["value"]) if cpu else None,
+        "speedup_vs_cpu_one_core": (rate / cpu["single_core"]) if cpu else None,
+        "batched": batched,
+    }
+    print(json.dumps(line))
+
+
+def batched_replicas(sn, rc):
+    reps = []
+    k = 0
+    for i in range(32):
+        T = 50.0 + 450.0 * i / 31.0
+        for Ez in (0.0, 0.01, 0.05):
+            reps.append(sn.ReplicaParams(32, 32, 32, DipoleCutOff=rc, T=T, Efield=(0, 0, Ez), seed=1000 + k))
+            k += 1
+    for i in range(32):
+        Lk = 16 + round(32 * i / 31)
+        reps.append(sn.ReplicaParams(Lk, Lk, Lk, DipoleCutOff=rc, T=300, seed=1000 + k))
+        k += 1
+    return reps
+
+
+def run_batched(sn, rc, log, sweeps=20):
+    reps = batched_replicas(sn, rc)
+    total = sum(r.N for r in reps)
+    ctx = sn.StarryNight(reps)
+    for r in range(len(reps)):
+        ctx.init_lattice_random(r)
+    ctx.MC_sweeps(3)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.MC_sweeps(sweeps, sync=False)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    agg = sweeps * total / dt
+    cfg = ctx.sweep_config()
+    ctx.close()
+    one = sn.StarryNight(sn.ReplicaParams(32, 32, 32, DipoleCutOff=rc, T=300, seed=7))
+    one.init_lattice_random(0)
+    one.MC_sweeps(3)
+    t0 = time.perf_counter()
+    one.MC_sweeps(sweeps * 4, sync=False)
+    one.sync()
+    dt1 = time.perf_counter() - t0
+    single = sweeps * 4 * 32 ** 3 / dt1
+    one.close()
+    log(f"[bench] batched {len(reps)} replicas ({total} sites): {agg:.4e}/s; single 32^3: {single:.4e}/s; "
+        f"ratio {agg / single:.1f}")
+    return {"replicas": len(reps), "sites_per_sweep": total, "aggregate": agg, "single_32cube": single,
+            "ratio_vs_single_32cube": agg / single, "sweeps": sweeps, "tiles": cfg["tiles"]}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,789 @@
+// sn_api.hip -- C ABI (include/starrynight_amd.h) over the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/starrynight_amd.h"
+#include "sn_kernels.h"
+#include "sn_schedule.h"
+
+using namespace sn;
+
+namespace {
+
+struct RepHost {
+    int X, Y, Z;
+    long long N, base;
+    int mixed;
+    double L0;
+};
+
+typedef void (*SweepFn)(SweepArgs, int, int);
+
+}  // namespace
+
+struct sn_ctx {
+    int R = 0, rc = 0;
+    bool two_d = false;
+    std::vector<sn_params> params;
+    std::vector<RepHost> reps;
+    std::vector<ReplicaGeom> geom;
+    Schedule sched;
+    sn_tuning tuning{};
+    int G = 16, NPL = 4, npairs = 0, noff = 0;
+    long long total_sites = 0;
+    size_t lds_bytes = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // device buffers
+    double4 *d_lat = nullptr;
+    RepDev *d_reps = nullptr;
+    TileDev *d_tiles = nullptr;
+    ShapeDev *d_shapes = nullptr;
+    int *d_class_start = nullptr, *d_class_sites = nullptr, *d_chunk_col = nullptr;
+    PairDev *d_pairs = nullptr, *d_offs = nullptr;
+    unsigned *d_sweep = nullptr;
+    unsigned long long *d_counters = nullptr;
+    double *d_partials = nullptr, *d_result = nullptr;
+    size_t partial_cap = 0;
+    std::vector<unsigned long long> sweeps_done;
+    // graph of one sweep
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_mixed = -1;
+    std::string err;
+};
+
+static int fail(sn_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, SN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---- sweep kernel instantiations ------------------------------------------
+#define SN_SWEEP_LIST(X) \
+    X(4, 4) X(4, 8) X(4, 16) X(8, 2) X(8, 4) X(8, 8) X(8, 16) X(16, 2) X(16, 4) X(16, 8) X(16, 16) \
+    X(32, 2) X(32, 4) X(32, 8)
+
+static SweepFn pick_sweep(int G, int NPL, bool mixed)
+{
+#define SN_CASE(g, n)                                                                  \
+    if (G == g && NPL == n) return mixed ? (SweepFn)tile_sweep_kernel<g, n, true>     \
+                                         : (SweepFn)tile_sweep_kernel<g, n, false>;
+    SN_SWEEP_LIST(SN_CASE)
+#undef SN_CASE
+    return nullptr;
+}
+
+static bool any_mixed(const sn_ctx *c)
+{
+    for (const RepHost &r : c->reps)
+        if (r.mixed) return true;
+    return false;
+}
+
+static double beta_of(double T)
+{
+    // beta = 1/((float)T/300.0) for integer T (src/starrynight-main.c:215);
+    // for a real-valued T the same ratio in double.
+    if (T == 0.0) return INFINITY;
+    if (T == std::floor(T) && std::fabs(T) < 16777216.0) return 1.0 / ((double)(float)T / 300.0);
+    return 300.0 / T;
+}
+
+static RepDev make_repdev(const sn_params &p, const RepHost &h)
+{
+    RepDev d{};
+    d.X = h.X; d.Y = h.Y; d.Z = h.Z; d.mixed = h.mixed;
+    d.base = h.base;
+    d.beta = beta_of(p.temperature);
+    d.J = p.cage_strain; d.K = p.K; d.L0 = h.L0;
+    d.E[0] = p.efield[0]; d.E[1] = p.efield[1]; d.E[2] = p.efield[2];
+    d.seed = p.seed;
+    d.constrain_to_x = p.constrain_to_x ? 1 : 0;
+    return d;
+}
+
+static int upload_rep(sn_ctx *c, int r)
+{
+    RepDev d = make_repdev(c->params[r], c->reps[r]);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_reps + r, &d, sizeof(d), hipMemcpyHostToDevice));
+    return SN_OK;
+}
+
+static void free_graph(sn_ctx *c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_mixed = -1;
+}
+
+template <typename T>
+static int upload(sn_ctx *c, T **dst, const std::vector<T> &v)
+{
+    size_t n = std::max<size_t>(v.size(), 1);
+    HIPCHK(c, hipMalloc((void **)dst, n * sizeof(T)));
+    if (!v.empty()) HIPCHK(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return SN_OK;
+}
+
+static int ensure_partials(sn_ctx *c, size_t n)
+{
+    if (n <= c->partial_cap) return SN_OK;
+    if (c->d_partials) (void)hipFree(c->d_partials);
+    c->d_partials = nullptr;
+    HIPCHK(c, hipMalloc((void **)&c->d_partials, n * sizeof(double)));
+    c->partial_cap = n;
+    return SN_OK;
+}
+
+static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &t, Schedule &s)
+{
+    int target[3] = {t.tile[0] > 0 ? t.tile[0] : 8, t.tile[1] > 0 ? t.tile[1] : 8,
+                     t.tile[2] > 0 ? t.tile[2] : 8};
+    bool two_d = geom[0].Z == 1;
+    int kind = t.fine_colouring;
+    if (kind == 0) kind = (!two_d && rc == 3) ? 3 : 2;
+    return build_schedule(geom, rc, target, kind, kBlock, s);
+}
+
+extern "C" {
+
+int sn_plan_schedule(const sn_params *params, int32_t n_replicas, const sn_tuning *tuning,
+                     int32_t replica, int64_t *n_sites, int32_t *n_phases, int32_t *order,
+                     int32_t *phase_start)
+{
+    if (!params || n_replicas < 1 || replica < 0 || replica >= n_replicas) return SN_ERR_INVALID;
+    std::vector<ReplicaGeom> geom;
+    for (int r = 0; r < n_replicas; r++) {
+        if (params[r].dipole_cutoff != params[0].dipole_cutoff) return SN_ERR_INVALID;
+        if (params[r].X < 1 || params[r].Y < 1 || params[r].Z < 1) return SN_ERR_INVALID;
+        geom.push_back({params[r].X, params[r].Y, params[r].Z});
+    }
+    sn_tuning t{};
+    if (tuning) t = *tuning;
+    Schedule s;
+    if (!plan(geom, params[0].dipole_cutoff, t, s)) return SN_ERR_UNSUPPORTED;
+    std::vector<int32_t> o, ps;
+    sweep_order(s, geom[replica], replica, o, ps);
+    if (n_sites) *n_sites = (int64_t)o.size();
+    if (n_phases) *n_phases = (int32_t)ps.size() - 1;
+    if (order) std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
+    if (phase_start) std::memcpy(phase_start, ps.data(), ps.size() * sizeof(int32_t));
+    return SN_OK;
+}
+
+const char *sn_status_string(int status)
+{
+    switch (status) {
+    case SN_OK: return "ok";
+    case SN_ERR_INVALID: return "invalid argument";
+    case SN_ERR_HIP: return "HIP runtime error";
+    case SN_ERR_NOMEM: return "out of memory";
+    case SN_ERR_RANGE: return "index out of range";
+    case SN_ERR_UNSUPPORTED: return "unsupported configuration";
+    }
+    return "unknown status";
+}
+
+const char *sn_last_error(const sn_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuning, sn_ctx **out)
+{
+    if (!out) return SN_ERR_INVALID;
+    *out = nullptr;
+    if (!params || n_replicas < 1) return SN_ERR_INVALID;
+    sn_ctx *c = new (std::nothrow) sn_ctx();
+    if (!c) return SN_ERR_NOMEM;
+    *out = c;
+    c->R = n_replicas;
+    c->params.assign(params, params + n_replicas);
+    c->rc = params[0].dipole_cutoff;
+    if (tuning) c->tuning = *tuning;
+    for (int r = 0; r < n_replicas; r++) {
+        const sn_params &p = params[r];
+        if (p.dipole_cutoff != c->rc)
+            return fail(c, SN_ERR_INVALID, "all replicas of a context must share DipoleCutOff");
+        if (p.X < 1 || p.Y < 1 || p.Z < 1) return fail(c, SN_ERR_INVALID, "lattice size must be positive");
+        if (!(p.temperature >= 0.0)) return fail(c, SN_ERR_INVALID, "temperature must be >= 0");
+        long long N = (long long)p.X * p.Y * p.Z;
+        if (N > 0x7fffffffLL) return fail(c, SN_ERR_RANGE, "lattice larger than 2^31 sites");
+        RepHost h{p.X, p.Y, p.Z, N, c->total_sites, 0, 1.0};
+        c->reps.push_back(h);
+        c->geom.push_back({p.X, p.Y, p.Z});
+        c->total_sites += N;
+    }
+    c->two_d = params[0].Z == 1;
+    if (!plan(c->geom, c->rc, c->tuning, c->sched)) return fail(c, SN_ERR_UNSUPPORTED, c->sched.error);
+
+    // neighbour table: +-r pairs for the tile sweep, full offsets for gathers
+    std::vector<Offset> pairs = neighbour_pairs(c->rc, c->two_d);
+    std::vector<Offset> offs = neighbour_offsets(c->rc, c->two_d);
+    auto to_dev = [](const Offset &o) {
+        PairDev p{};
+        p.dx = o.dx; p.dy = o.dy; p.dz = o.dz; p.nn = (o.r2 == 1);
+        double d = std::sqrt((double)o.r2);
+        p.nx = o.dx / d; p.ny = o.dy / d; p.nz = o.dz / d;
+        p.w = 1.0 / (d * d * d);
+        return p;
+    };
+    std::vector<PairDev> hp, ho;
+    for (const Offset &o : pairs) hp.push_back(to_dev(o));
+    for (const Offset &o : offs) ho.push_back(to_dev(o));
+    c->npairs = (int)hp.size();
+    c->noff = (int)ho.size();
+
+    // lanes per site and pairs per lane
+    int max_class = 1;
+    for (const TileShape &s : c->sched.shapes) max_class = std::max(max_class, s.max_class);
+    int G = c->tuning.lanes_per_site;
+    if (G <= 0) {
+        G = 32;
+        while (G > 4 && (kBlock / G) < max_class) G >>= 1;
+    }
+    int need = (c->npairs + G - 1) / G;
+    int NPL = 2;
+    while (NPL < need) NPL <<= 1;
+    if (!pick_sweep(G, NPL, false))
+        return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
+    c->G = G; c->NPL = NPL;
+
+    HIPCHK(c, hipGetDevice(&c->device));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipMalloc((void **)&c->d_lat, (size_t)c->total_sites * sizeof(double4)));
+    HIPCHK(c, hipMemset(c->d_lat, 0, (size_t)c->total_sites * sizeof(double4)));
+    std::vector<RepDev> rd;
+    for (int r = 0; r < c->R; r++) rd.push_back(make_repdev(c->params[r], c->reps[r]));
+    int st;
+    if ((st = upload(c, &c->d_reps, rd))) return st;
+    std::vector<TileDev> td;
+    for (const Tile &t : c->sched.tiles) td.push_back({t.replica, t.x0, t.y0, t.z0, t.shape, 0, 0, 0});
+    if ((st = upload(c, &c->d_tiles, td))) return st;
+    std::vector<ShapeDev> sd;
+    std::vector<int> cstart, csites, chunks;
+    for (const TileShape &s : c->sched.shapes) {
+        ShapeDev d{};
+        d.tx = s.tx; d.ty = s.ty; d.tz = s.tz; d.ncol = s.ncol;
+        d.nchunks = (int)s.chunk_start.size() - 1;
+        d.class_off = (int)cstart.size();
+        d.chunk_off = (int)chunks.size();
+        d.sites_off = (int)csites.size();
+        cstart.insert(cstart.end(), s.class_start.begin(), s.class_start.end());
+        csites.insert(csites.end(), s.class_sites.begin(), s.class_sites.end());
+        chunks.insert(chunks.end(), s.chunk_start.begin(), s.chunk_start.end());
+        sd.push_back(d);
+    }
+    if ((st = upload(c, &c->d_shapes, sd))) return st;
+    if ((st = upload(c, &c->d_class_start, cstart))) return st;
+    if ((st = upload(c, &c->d_class_sites, csites))) return st;
+    if ((st = upload(c, &c->d_chunk_col, chunks))) return st;
+    if ((st = upload(c, &c->d_pairs, hp))) return st;
+    if ((st = upload(c, &c->d_offs, ho))) return st;
+    std::vector<unsigned> zeros_u(c->R, 0u);
+    if ((st = upload(c, &c->d_sweep, zeros_u))) return st;
+    std::vector<unsigned long long> zeros_c(2 * c->R, 0ull);
+    if ((st = upload(c, &c->d_counters, zeros_c))) return st;
+    HIPCHK(c, hipMalloc((void **)&c->d_result, 64 * sizeof(double)));
+    c->sweeps_done.assign(c->R, 0);
+
+    // LDS: 3 (or 4) region arrays + 4 x kBlock proposal slots
+    size_t mixed_lds = (size_t)(4 * c->sched.max_region + 4 * kBlock) * sizeof(double);
+    c->lds_bytes = mixed_lds;
+    if (mixed_lds > 160 * 1024)
+        return fail(c, SN_ERR_UNSUPPORTED, "tile + halo does not fit in LDS; use a smaller tile");
+    for (int m = 0; m < 2; m++) {
+        SweepFn f = pick_sweep(G, NPL, m == 1);
+        HIPCHK(c, hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)mixed_lds));
+    }
+    return SN_OK;
+}
+
+int sn_destroy(sn_ctx *c)
+{
+    if (!c) return SN_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_graph(c);
+    void *bufs[] = {c->d_lat, c->d_reps, c->d_tiles, c->d_shapes, c->d_class_start, c->d_class_sites,
+                    c->d_chunk_col, c->d_pairs, c->d_offs, c->d_sweep, c->d_counters, c->d_partials,
+                    c->d_result};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SN_OK;
+}
+
+static int check_rep(sn_ctx *c, int r)
+{
+    if (!c) return SN_ERR_INVALID;
+    if (r < 0 || r >= c->R) return fail(c, SN_ERR_RANGE, "replica index out of range");
+    return SN_OK;
+}
+
+int sn_set_lattice(sn_ctx *c, int32_t r, const double *xyzl)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!xyzl) return fail(c, SN_ERR_INVALID, "null lattice buffer");
+    RepHost &h = c->reps[r];
+    double L0 = xyzl[3];
+    int mixed = 0;
+    for (long long i = 0; i < h.N; i++) {
+        const double *p = xyzl + 4 * i;
+        if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]) || !std::isfinite(p[3]))
+            return fail(c, SN_ERR_INVALID, "non-finite value in lattice");
+        if (p[3] != L0) mixed = 1;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_lat + h.base, xyzl, h.N * sizeof(double4), hipMemcpyHostToDevice));
+    h.mixed = mixed;
+    h.L0 = mixed ? 0.0 : L0;
+    return upload_rep(c, r);
+}
+
+int sn_get_lattice(sn_ctx *c, int32_t r, double *xyzl)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!xyzl) return fail(c, SN_ERR_INVALID, "null lattice buffer");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(xyzl, c->d_lat + c->reps[r].base, c->reps[r].N * sizeof(double4),
+                        hipMemcpyDeviceToHost));
+    return SN_OK;
+}
+
+int sn_lattice_device_ptr(sn_ctx *c, int32_t r, void **p)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!p) return fail(c, SN_ERR_INVALID, "null output pointer");
+    *p = (void *)(c->d_lat + c->reps[r].base);
+    return SN_OK;
+}
+
+int sn_init_lattice_random(sn_ctx *c, int32_t r, int32_t n_species, const double *lengths,
+                           const double *prevalence)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    std::vector<double> len(1, 1.0), cum(1, 1.0);
+    if (n_species > 0) {
+        if (n_species > 10 || !lengths || !prevalence)
+            return fail(c, SN_ERR_INVALID, "species table must have 1..10 entries");
+        len.assign(lengths, lengths + n_species);
+        cum.assign(n_species, 0.0);
+        double acc = 0.0;
+        for (int i = 0; i < n_species; i++) {
+            if (!(prevalence[i] >= 0.0)) return fail(c, SN_ERR_INVALID, "negative prevalence");
+            acc += prevalence[i];
+            cum[i] = acc;
+        }
+        if (acc < 1.0 - 1e-6) return fail(c, SN_ERR_INVALID, "Prevalence must sum to 1");
+    }
+    RepHost &h = c->reps[r];
+    // lengths that can occur: species with non-zero prevalence
+    std::vector<double> used;
+    for (size_t i = 0; i < len.size(); i++)
+        if ((i == 0 ? cum[0] : cum[i] - cum[i - 1]) > 0.0) used.push_back(len[i]);
+    if (used.empty()) used.push_back(len[0]);
+    h.mixed = 0;
+    for (double v : used)
+        if (v != used[0]) h.mixed = 1;
+    h.L0 = h.mixed ? 0.0 : used[0];
+    if ((st = upload_rep(c, r))) return st;
+    double *dl = nullptr, *dc = nullptr;
+    HIPCHK(c, hipMalloc((void **)&dl, len.size() * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&dc, cum.size() * sizeof(double)));
+    HIPCHK(c, hipMemcpy(dl, len.data(), len.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dc, cum.data(), cum.size() * sizeof(double), hipMemcpyHostToDevice));
+    int nblk = (int)((h.N + 255) / 256);
+    hipLaunchKernelGGL(init_random_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
+                       (int)len.size(), dl, dc);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(dl);
+    (void)hipFree(dc);
+    return SN_OK;
+}
+
+int sn_set_params(sn_ctx *c, int32_t r, double temperature, const double efield[3], double cage_strain)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!(temperature >= 0.0)) return fail(c, SN_ERR_INVALID, "temperature must be >= 0");
+    sn_params &p = c->params[r];
+    p.temperature = temperature;
+    if (efield) { p.efield[0] = efield[0]; p.efield[1] = efield[1]; p.efield[2] = efield[2]; }
+    p.cage_strain = cage_strain;
+    return upload_rep(c, r);
+}
+
+int sn_site_energy_batch_device(sn_ctx *c, int32_t r, int64_t n, const int32_t *site,
+                                const double *prop, double *dE)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (n < 0) return fail(c, SN_ERR_INVALID, "negative count");
+    if (n == 0) return SN_OK;
+    if (!site || !prop || !dE) return fail(c, SN_ERR_INVALID, "null buffer");
+    long long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(site_energy_batch_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, c->d_lat,
+                       c->d_reps, r, c->d_offs, c->noff, (long long)n, site, prop, dE);
+    HIPCHK(c, hipGetLastError());
+    return SN_OK;
+}
+
+int sn_site_energy_batch(sn_ctx *c, int32_t r, int64_t n, const int32_t *site, const double *prop,
+                         double *dE)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (n < 0) return fail(c, SN_ERR_INVALID, "negative count");
+    if (n == 0) return SN_OK;
+    if (!site || !prop || !dE) return fail(c, SN_ERR_INVALID, "null buffer");
+    for (int64_t k = 0; k < n; k++)
+        if (site[k] < 0 || site[k] >= c->reps[r].N) return fail(c, SN_ERR_RANGE, "site index out of range");
+    int32_t *ds = nullptr;
+    double *dp = nullptr, *de = nullptr;
+    HIPCHK(c, hipMalloc((void **)&ds, n * sizeof(int32_t)));
+    HIPCHK(c, hipMalloc((void **)&dp, n * 3 * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&de, n * sizeof(double)));
+    HIPCHK(c, hipMemcpy(ds, site, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dp, prop, n * 3 * sizeof(double), hipMemcpyHostToDevice));
+    st = sn_site_energy_batch_device(c, r, n, ds, dp, de);
+    if (st == SN_OK) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(dE, de, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    (void)hipFree(ds); (void)hipFree(dp); (void)hipFree(de);
+    return st;
+}
+
+static SweepArgs sweep_args(sn_ctx *c)
+{
+    SweepArgs a;
+    a.lat = c->d_lat; a.reps = c->d_reps; a.tiles = c->d_tiles; a.shapes = c->d_shapes;
+    a.class_start = c->d_class_start; a.class_sites = c->d_class_sites; a.chunk_col = c->d_chunk_col;
+    a.pairs = c->d_pairs; a.sweep_ctr = c->d_sweep; a.counters = c->d_counters;
+    a.npairs = c->npairs; a.rc = c->rc; a.two_d = c->two_d ? 1 : 0;
+    a.max_region = c->sched.max_region;
+    return a;
+}
+
+static size_t sweep_lds(const sn_ctx *c, bool mixed)
+{
+    return (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * kBlock) * sizeof(double);
+}
+
+// Enqueue one sweep: one tile-sweep launch per non-empty coarse colour, then
+// the sweep-counter bump.  events (optional) bracket each tile-sweep launch.
+static int enqueue_sweep(sn_ctx *c, std::vector<hipEvent_t> *events)
+{
+    const bool mixed = any_mixed(c);
+    SweepFn f = pick_sweep(c->G, c->NPL, mixed);
+    SweepArgs a = sweep_args(c);
+    size_t lds = sweep_lds(c, mixed);
+    for (int cc = 0; cc < 8; cc++) {
+        int first = c->sched.coarse_start[cc], n = c->sched.coarse_start[cc + 1] - first;
+        if (n <= 0) continue;
+        if (events) {
+            hipEvent_t e0, e1;
+            HIPCHK(c, hipEventCreate(&e0));
+            HIPCHK(c, hipEventCreate(&e1));
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            hipLaunchKernelGGL(f, dim3(n), dim3(kBlock), lds, c->stream, a, first, n);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            events->push_back(e0);
+            events->push_back(e1);
+        } else {
+            hipLaunchKernelGGL(f, dim3(n), dim3(kBlock), lds, c->stream, a, first, n);
+            HIPCHK(c, hipGetLastError());
+        }
+    }
+    hipLaunchKernelGGL(bump_sweep_kernel, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, c->d_sweep, c->R);
+    HIPCHK(c, hipGetLastError());
+    return SN_OK;
+}
+
+static int build_graph(sn_ctx *c)
+{
+    free_graph(c);
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int st = enqueue_sweep(c, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+    if (e != hipSuccess) return fail(c, SN_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    c->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    c->graph_mixed = any_mixed(c) ? 1 : 0;
+    return SN_OK;
+}
+
+int sn_mc_sweeps(sn_ctx *c, int32_t nsweeps)
+{
+    if (!c) return SN_ERR_INVALID;
+    if (nsweeps < 0) return fail(c, SN_ERR_INVALID, "negative sweep count");
+    const bool use_graph = c->tuning.use_graph >= 0;  // 0 (auto) and 1 capture a graph
+    if (use_graph) {
+        int want = any_mixed(c) ? 1 : 0;
+        if (!c->graph_exec || c->graph_mixed != want) {
+            int st = build_graph(c);
+            if (st) return st;
+        }
+        for (int i = 0; i < nsweeps; i++) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    } else {
+        for (int i = 0; i < nsweeps; i++) {
+            int st = enqueue_sweep(c, nullptr);
+            if (st) return st;
+        }
+    }
+    for (int r = 0; r < c->R; r++) c->sweeps_done[r] += (unsigned long long)nsweeps;
+    return SN_OK;
+}
+
+int sn_sync(sn_ctx *c)
+{
+    if (!c) return SN_ERR_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SN_OK;
+}
+
+int sn_time_sweep_kernel(sn_ctx *c, int32_t nsweeps, double *mean_ms, int32_t *launches)
+{
+    if (!c || nsweeps < 1 || !mean_ms) return fail(c, SN_ERR_INVALID, "bad arguments");
+    std::vector<hipEvent_t> ev;
+    int st = SN_OK;
+    for (int i = 0; i < nsweeps && st == SN_OK; i++) st = enqueue_sweep(c, &ev);
+    if (st) return st;
+    for (int r = 0; r < c->R; r++) c->sweeps_done[r] += (unsigned long long)nsweeps;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        tot += ms;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    int nl = (int)(ev.size() / 2);
+    *mean_ms = nl ? tot / nl : 0.0;
+    if (launches) *launches = nl;
+    return SN_OK;
+}
+
+int sn_mc_replay(sn_ctx *c, int32_t r, int64_t n, const int32_t *site, const double *prop,
+                 const double *u, const uint8_t *u_drawn, const uint8_t *ref_accept,
+                 uint8_t *decision_out, uint64_t *mismatches, uint64_t *near_ties)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (n < 0) return fail(c, SN_ERR_INVALID, "negative count");
+    if (n == 0) { if (mismatches) *mismatches = 0; if (near_ties) *near_ties = 0; return SN_OK; }
+    if (!site || !prop || !u || !u_drawn || !ref_accept || !decision_out)
+        return fail(c, SN_ERR_INVALID, "null buffer");
+    for (int64_t k = 0; k < n; k++)
+        if (site[k] < 0 || site[k] >= c->reps[r].N) return fail(c, SN_ERR_RANGE, "site index out of range");
+    int32_t *ds = nullptr;
+    double *dp = nullptr, *du = nullptr;
+    uint8_t *dd = nullptr, *da = nullptr, *dout = nullptr;
+    unsigned long long *dstat = nullptr;
+    HIPCHK(c, hipMalloc((void **)&ds, n * sizeof(int32_t)));
+    HIPCHK(c, hipMalloc((void **)&dp, n * 3 * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&du, n * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&dd, n));
+    HIPCHK(c, hipMalloc((void **)&da, n));
+    HIPCHK(c, hipMalloc((void **)&dout, n));
+    HIPCHK(c, hipMalloc((void **)&dstat, 4 * sizeof(unsigned long long)));
+    HIPCHK(c, hipMemcpy(ds, site, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dp, prop, n * 3 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(du, u, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dd, u_drawn, n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(da, ref_accept, n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    hipLaunchKernelGGL(replay_kernel, dim3(1), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r, c->d_offs,
+                       c->noff, (long long)n, ds, dp, du, dd, da, dout, dstat);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long stats[4];
+    HIPCHK(c, hipMemcpy(decision_out, dout, n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(stats, dstat, sizeof(stats), hipMemcpyDeviceToHost));
+    if (mismatches) *mismatches = stats[0];
+    if (near_ties) *near_ties = stats[1];
+    // the replay's decisions count toward the replica's counters
+    unsigned long long hc[2];
+    HIPCHK(c, hipMemcpy(hc, c->d_counters + 2 * r, sizeof(hc), hipMemcpyDeviceToHost));
+    hc[0] += stats[2]; hc[1] += stats[3];
+    HIPCHK(c, hipMemcpy(c->d_counters + 2 * r, hc, sizeof(hc), hipMemcpyHostToDevice));
+    void *bufs[] = {ds, dp, du, dd, da, dout, dstat};
+    for (void *b : bufs) (void)hipFree(b);
+    return SN_OK;
+}
+
+int sn_counters(sn_ctx *c, int32_t r, uint64_t *accept, uint64_t *reject, uint64_t *attempted)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    unsigned long long hc[2];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(hc, c->d_counters + 2 * r, sizeof(hc), hipMemcpyDeviceToHost));
+    if (accept) *accept = hc[0];
+    if (reject) *reject = hc[1];
+    if (attempted) *attempted = c->sweeps_done[r] * (unsigned long long)c->reps[r].N;
+    return SN_OK;
+}
+
+int sn_reset_counters(sn_ctx *c)
+{
+    if (!c) return SN_ERR_INVALID;
+    std::vector<unsigned long long> z(2 * c->R, 0ull);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_counters, z.data(), z.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    std::fill(c->sweeps_done.begin(), c->sweeps_done.end(), 0ull);
+    return SN_OK;
+}
+
+int sn_gradient(sn_ctx *c, int32_t r, double *grad_p, double dparams[6], double *energy)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    const RepHost &h = c->reps[r];
+    int nblk = (int)((h.N + 255) / 256);
+    if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
+    double *dg = nullptr;
+    if (grad_p) HIPCHK(c, hipMalloc((void **)&dg, h.N * 3 * sizeof(double)));
+    hipLaunchKernelGGL(energy_grad_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
+                       c->d_offs, c->noff, dg, c->d_partials);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk, kNPart,
+                       c->d_result);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double res[kNPart];
+    HIPCHK(c, hipMemcpy(res, c->d_result, sizeof(res), hipMemcpyDeviceToHost));
+    if (grad_p) {
+        HIPCHK(c, hipMemcpy(grad_p, dg, h.N * 3 * sizeof(double), hipMemcpyDeviceToHost));
+        (void)hipFree(dg);
+    }
+    if (dparams) for (int i = 0; i < 6; i++) dparams[i] = res[4 + i];
+    if (energy) {
+        // compensated sum of the four parts
+        double s = 0.0, cmp = 0.0;
+        for (int i = 0; i < 4; i++) {
+            double t = s + res[i];
+            cmp += (std::fabs(s) >= std::fabs(res[i])) ? (s - t) + res[i] : (res[i] - t) + s;
+            s = t;
+        }
+        *energy = s + cmp;
+    }
+    return SN_OK;
+}
+
+int sn_energy(sn_ctx *c, int32_t r, double *energy, double parts[4])
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    const RepHost &h = c->reps[r];
+    int nblk = (int)((h.N + 255) / 256);
+    if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
+    hipLaunchKernelGGL(energy_grad_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
+                       c->d_offs, c->noff, (double *)nullptr, c->d_partials);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk, kNPart,
+                       c->d_result);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double res[kNPart];
+    HIPCHK(c, hipMemcpy(res, c->d_result, sizeof(res), hipMemcpyDeviceToHost));
+    if (parts) for (int i = 0; i < 4; i++) parts[i] = res[i];
+    if (energy) {
+        double s = 0.0, cmp = 0.0;
+        for (int i = 0; i < 4; i++) {
+            double t = s + res[i];
+            cmp += (std::fabs(s) >= std::fabs(res[i])) ? (s - t) + res[i] : (res[i] - t) + s;
+            s = t;
+        }
+        *energy = s + cmp;
+    }
+    return SN_OK;
+}
+
+int sn_moment(sn_ctx *c, int32_t r, double M[3])
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!M) return fail(c, SN_ERR_INVALID, "null output");
+    const RepHost &h = c->reps[r];
+    int nblk = (int)std::min<long long>((h.N + 255) / 256, 1024);
+    if ((st = ensure_partials(c, (size_t)nblk * 3))) return st;
+    hipLaunchKernelGGL(moment_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r, c->d_partials);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk, 3,
+                       c->d_result);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(M, c->d_result, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return SN_OK;
+}
+
+int sn_schedule_info(sn_ctx *c, int32_t r, int64_t *n_sites, int32_t *n_phases)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    std::vector<int32_t> order, ps;
+    sweep_order(c->sched, c->geom[r], r, order, ps);
+    if (n_sites) *n_sites = (int64_t)order.size();
+    if (n_phases) *n_phases = (int32_t)ps.size() - 1;
+    return SN_OK;
+}
+
+int sn_schedule_order(sn_ctx *c, int32_t r, int32_t *order, int32_t *phase_start)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    std::vector<int32_t> o, ps;
+    sweep_order(c->sched, c->geom[r], r, o, ps);
+    if (order) std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
+    if (phase_start) std::memcpy(phase_start, ps.data(), ps.size() * sizeof(int32_t));
+    return SN_OK;
+}
+
+int sn_stream(sn_ctx *c, void **s)
+{
+    if (!c || !s) return SN_ERR_INVALID;
+    *s = (void *)c->stream;
+    return SN_OK;
+}
+
+int sn_sweep_config(sn_ctx *c, int32_t *tiles, int32_t *launches, int32_t *lanes, int32_t *fine)
+{
+    if (!c) return SN_ERR_INVALID;
+    int nl = 0;
+    for (int cc = 0; cc < 8; cc++)
+        if (c->sched.coarse_start[cc + 1] > c->sched.coarse_start[cc]) nl++;
+    int maxcol = 0;
+    for (const TileShape &s : c->sched.shapes) maxcol = std::max(maxcol, s.ncol);
+    if (tiles) *tiles = (int32_t)c->sched.tiles.size();
+    if (launches) *launches = nl;
+    if (lanes) *lanes = c->G;
+    if (fine) *fine = maxcol;
+    return SN_OK;
+}
+
+}  // extern "C"

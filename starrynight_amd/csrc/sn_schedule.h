@@ -1,0 +1,89 @@
+// sn_schedule.h -- host-side colour schedule for the tile sweep (pure C++).
+//
+// A sweep visits every site once.  Tiles (blocks of sites, edge >= rc) get one
+// of up to 8 coarse colours by the parity of their tile coordinates; tiles of
+// one coarse colour are separated by a full tile (>= rc sites) along some axis,
+// so they never interact and run concurrently, one workgroup each.  Inside a
+// tile the sites are split into fine colours by a sublattice whose shortest
+// vector is longer than rc; one fine colour is updated at a time with
+// workgroup barriers in between.  The neighbour table follows gen_neighbour
+// (src/starrynight-montecarlo-core.c:38-72): every r != 0 with |r| <= rc,
+// dz == 0 when Z == 1.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace sn {
+
+struct Offset {
+    int dx, dy, dz;
+    int r2;
+};
+
+// Neighbour offsets in gen_neighbour order (dx, dy, dz ascending).
+std::vector<Offset> neighbour_offsets(int rc, bool two_d);
+// One representative r of every +-r pair, nearest neighbours (|r| = 1) first.
+std::vector<Offset> neighbour_pairs(int rc, bool two_d);
+
+// Sublattice used for the fine colouring, as a lower-triangular basis
+// v1 = (a,0,0), v2 = (b,d,0), v3 = (c,e,f); colours = a*d*f.
+struct Sublattice {
+    int a, b, c, d, e, f;
+    int colours() const { return a * d * f; }
+    int colour(int x, int y, int z) const;
+    int min_norm2() const;
+    std::string describe() const;
+};
+// kind: 1 cubic (rc+1)^3, 2 minimum index with shortest vector^2 > rc^2,
+// 3 body-centred (4,0,0),(0,4,0),(2,2,2) (valid for rc <= 3).
+Sublattice make_sublattice(int rc, bool two_d, int kind);
+
+// Split an axis of length L into an even number of tiles, each of size >= rc,
+// near `target`, with tile + 2*rc <= L (so a tile's halo never wraps onto
+// itself).  L == 1 (the z axis of a 2D lattice) gives one tile of size 1.
+// Returns false when no such split exists.
+bool split_axis(int L, int rc, int target, std::vector<int> &start, std::vector<int> &size);
+
+struct TileShape {
+    int tx, ty, tz;             // tile size
+    int ncol;                   // fine colours
+    int max_class;              // largest fine class
+    std::vector<int> class_start;  // ncol + 1
+    std::vector<int> class_sites;  // tile-local linear index (lx*ty + ly)*tz + lz, by colour
+    std::vector<int> chunk_start;  // colour index where each proposal chunk starts (+ ncol)
+};
+
+struct Tile {
+    int replica;
+    int x0, y0, z0;
+    int shape;
+    int coarse;   // 0..7
+};
+
+struct ReplicaGeom {
+    int X, Y, Z;
+};
+
+struct Schedule {
+    int rc = 0;
+    bool two_d = false;
+    Sublattice lat{};
+    std::vector<TileShape> shapes;
+    std::vector<Tile> tiles;                       // grouped by coarse colour
+    std::vector<int> coarse_start;                 // 9 entries
+    int max_region = 0;                            // largest (tx+2h)(ty+2h)(tz+2hz)
+    std::string error;
+};
+
+// Build the schedule for all replicas.  target[3] = tile edge aimed for per
+// axis; chunk = block size (sites whose proposals are drawn at once).
+bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int target[3],
+                    int colouring_kind, int chunk, Schedule &out);
+
+// Visit order of replica r for one sweep and its phase boundaries
+// (phase = one (coarse colour, fine colour) pair across all tiles).
+void sweep_order(const Schedule &s, const ReplicaGeom &g, int replica, std::vector<int32_t> &order,
+                 std::vector<int32_t> &phase_start);
+
+}  // namespace sn

@@ -1,0 +1,136 @@
+"""Colour-sweep kernel (sn_mc_sweeps) vs the oracle's ordered counter-RNG sweep.
+
+The library reports its visit order (sn_schedule_order); the oracle replays
+the same order with the same Philox draws in FP64.  Bar: identical accept /
+reject counts and lattices equal to 1e-12 (the proposals are bit-reproducible;
+only the order of the dE sum differs).
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import oracle_model, print_case, rand_lattice
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "16cube_rc3_uniform": (dict(X=16, Y=16, Z=16, T=300), False, {}),
+    "16cube_rc3_mixed_field_K": (dict(X=16, Y=16, Z=16, T=250, Efield=(0.05, 0.0, 0.1), K=0.3), True, {}),
+    "25x25x1_rc3": (dict(X=25, Y=25, Z=1, T=300), False, {}),
+    "20cube_rc4_mixed": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300), True, {}),
+    "12x16x20_rc3": (dict(X=12, Y=16, Z=20, T=400, CageStrain=2.0), False, {}),
+    "16cube_constrain_x": (dict(X=16, Y=16, Z=16, T=300, ConstrainToX=True), False, {}),
+    "16cube_T0": (dict(X=16, Y=16, Z=16, T=0), False, {}),
+    "16cube_lanes8_minidx": (dict(X=16, Y=16, Z=16, T=300), True, dict(lanes_per_site=8, fine_colouring=2)),
+    "16cube_lanes32_cubic": (dict(X=16, Y=16, Z=16, T=300), False, dict(lanes_per_site=32, fine_colouring=1)),
+    "18cube_direct_launch": (dict(X=18, Y=18, Z=18, T=300), False, dict(use_graph=-1)),
+}
+
+
+def run_pair(sn, oracle, reps, lats, tuning, nsweeps):
+    tun = sn.TuningParams(**tuning) if tuning else None
+    with sn.StarryNight(reps, tun) as ctx:
+        for r, lat in enumerate(lats):
+            ctx.set_lattice(lat, r)
+        ctx.MC_sweeps(nsweeps)
+        got = [ctx.get_lattice(r) for r in range(len(reps))]
+        cnt = [ctx.counters(r) for r in range(len(reps))]
+        orders = [ctx.schedule(r)[0] for r in range(len(reps))]
+    for r, rp in enumerate(reps):
+        o = oracle_model(oracle, rp)
+        want = lats[r].copy()
+        acc = rej = 0
+        for s in range(nsweeps):
+            a, b = o.sweep_ordered(want, orders[r], rp.seed, r, s)
+            acc += a; rej += b
+        err = np.abs(got[r] - want).max()
+        print(f"  replica {r}: max |dlat| {err:.3e}  acc {cnt[r]['accept']}/{acc} rej {cnt[r]['reject']}/{rej}",
+              flush=True)
+        assert cnt[r]["accept"] == acc and cnt[r]["reject"] == rej
+        assert cnt[r]["attempted"] == nsweeps * rp.N
+        assert err <= 1e-12
+        assert np.array_equal(got[r][:, 3], lats[r][:, 3])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_sweep_parity(sn, oracle, name):
+    kw, mixed, tuning = CASES[name]
+    rp = sn.ReplicaParams(seed=0x5EED0000 + len(name), **kw)
+    print_case("sweep", name=name, **kw)
+    lat = rand_lattice(rp.N, np.random.default_rng(len(name)), mixed=mixed)
+    run_pair(sn, oracle, [rp], [lat], tuning, nsweeps=3)
+
+
+def test_batched_uneven_replicas_parity(sn, oracle):
+    """Replicas of different sizes, temperatures and fields in one context:
+    each follows exactly the trajectory the oracle gives it alone."""
+    reps = [sn.ReplicaParams(16, 16, 16, T=100, seed=11),
+            sn.ReplicaParams(18, 18, 18, T=300, Efield=(0, 0, 0.05), seed=12),
+            sn.ReplicaParams(24, 20, 17, T=500, seed=13),
+            sn.ReplicaParams(16, 16, 16, T=0, seed=14)]
+    print_case("batched_uneven", n=len(reps))
+    rng = np.random.default_rng(7)
+    lats = [rand_lattice(r.N, rng, mixed=(i == 1)) for i, r in enumerate(reps)]
+    run_pair(sn, oracle, reps, lats, {}, nsweeps=2)
+
+
+def test_batched_equals_single(sn):
+    """A replica's trajectory does not depend on what else is batched with it."""
+    a = sn.ReplicaParams(16, 16, 16, T=300, seed=99)
+    b = sn.ReplicaParams(20, 20, 20, T=150, seed=98)
+    lat = rand_lattice(a.N, np.random.default_rng(3))
+    with sn.StarryNight([a]) as c1:
+        c1.set_lattice(lat); c1.MC_sweeps(5); one = c1.get_lattice()
+    with sn.StarryNight([b, a]) as c2:
+        c2.set_lattice(rand_lattice(b.N, np.random.default_rng(4)), 0)
+        c2.set_lattice(lat, 1); c2.MC_sweeps(5); two = c2.get_lattice(1)
+    # same schedule for `a`, replica index differs -> different RNG stream unless
+    # the replica slot matches; compare with the replica in slot 0 instead
+    with sn.StarryNight([a, b]) as c3:
+        c3.set_lattice(lat, 0); c3.set_lattice(rand_lattice(b.N, np.random.default_rng(4)), 1)
+        c3.MC_sweeps(5); three = c3.get_lattice(0)
+    assert np.array_equal(one, three)
+    assert not np.array_equal(one, two)   # slot 1 draws a different stream
+
+
+def test_graph_replay_matches_direct_launch(sn):
+    rp = sn.ReplicaParams(16, 16, 16, T=300, seed=5)
+    lat = rand_lattice(rp.N, np.random.default_rng(5), mixed=True)
+    out = []
+    for g in (0, -1):
+        with sn.StarryNight(rp, sn.TuningParams(use_graph=g)) as ctx:
+            ctx.set_lattice(lat)
+            for _ in range(3):   # several replays with host work in between
+                ctx.MC_sweeps(2)
+                _ = ctx.counters()
+            out.append(ctx.get_lattice())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_set_params_between_sweeps(sn, oracle):
+    """Temperature / field ramps through sn_set_params reach the captured graph."""
+    rp = sn.ReplicaParams(16, 16, 16, T=300, seed=21)
+    lat = rand_lattice(rp.N, np.random.default_rng(21))
+    ramp = [(300, (0, 0, 0.0)), (200, (0, 0, 0.05)), (100, (0, 0, 0.1))]
+    with sn.StarryNight(rp) as ctx:
+        ctx.set_lattice(lat)
+        order = ctx.schedule()[0]
+        for T, E in ramp:
+            ctx.set_params(0, T, E, 1.0)
+            ctx.MC_sweeps(1)
+        got = ctx.get_lattice()
+    want = lat.copy()
+    for s, (T, E) in enumerate(ramp):
+        o = oracle_model(oracle, sn.ReplicaParams(16, 16, 16, T=T, Efield=E))
+        o.sweep_ordered(want, order, rp.seed, 0, s)
+    assert np.abs(got - want).max() <= 1e-12
+
+
+def test_device_random_init(sn):
+    rp = sn.ReplicaParams(32, 32, 32, seed=77)
+    with sn.StarryNight(rp) as ctx:
+        ctx.init_lattice_random(0, Dipoles=(1.0, 0.0), Prevalence=(0.5, 0.5))
+        lat = ctx.get_lattice()
+    assert np.abs(np.linalg.norm(lat[:, :3], axis=1) - 1).max() < 1e-15
+    frac = (lat[:, 3] == 1.0).mean()
+    assert abs(frac - 0.5) < 0.02 and set(np.unique(lat[:, 3])) == {0.0, 1.0}
+    assert np.abs(lat[:, :3].mean(0)).max() < 0.02
