@@ -45,12 +45,15 @@ struct sn_ctx {
     RepDev *d_reps = nullptr;
     TileDev *d_tiles = nullptr;
     ShapeDev *d_shapes = nullptr;
-    int *d_class_start = nullptr, *d_class_sites = nullptr, *d_chunk_col = nullptr;
+    int *d_class_start = nullptr, *d_class_ri = nullptr, *d_chunk_col = nullptr;
     PairDev *d_pairs = nullptr, *d_offs = nullptr;
     unsigned *d_sweep = nullptr;
     unsigned long long *d_counters = nullptr;
     double *d_partials = nullptr, *d_result = nullptr;
     size_t partial_cap = 0;
+    int max_tile_sites = 0, max_cls = 0, max_chunks = 0;
+    bool can_sweep = false;
+    std::string sweep_err;
     std::vector<unsigned long long> sweeps_done;
     // graph of one sweep
     hipGraphExec_t graph_exec = nullptr;
@@ -152,6 +155,10 @@ static int ensure_partials(sn_ctx *c, size_t n)
     return SN_OK;
 }
 
+// Largest tile + halo (sites) staged in LDS: 4 doubles per site (mixed lengths)
+// plus the proposal slots and shape tables stay under the 160 KB of a CU.
+constexpr int kMaxRegion = 4096;
+
 static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &t, Schedule &s)
 {
     int target[3] = {t.tile[0] > 0 ? t.tile[0] : 8, t.tile[1] > 0 ? t.tile[1] : 8,
@@ -159,7 +166,14 @@ static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &
     bool two_d = geom[0].Z == 1;
     int kind = t.fine_colouring;
     if (kind == 0) kind = (!two_d && rc == 3) ? 3 : 2;
-    return build_schedule(geom, rc, target, kind, kBlock, s);
+    return build_schedule(geom, rc, target, kind, kBlock, kMaxRegion, s);
+}
+
+static size_t sweep_lds(const sn_ctx *c, bool mixed)
+{
+    size_t dbl = (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * kBlock) * sizeof(double);
+    size_t ints = (size_t)(c->max_tile_sites + c->max_cls + 1 + c->max_chunks + 1) * sizeof(int);
+    return dbl + ints;
 }
 
 extern "C" {
@@ -220,6 +234,13 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         if (p.dipole_cutoff != c->rc)
             return fail(c, SN_ERR_INVALID, "all replicas of a context must share DipoleCutOff");
         if (p.X < 1 || p.Y < 1 || p.Z < 1) return fail(c, SN_ERR_INVALID, "lattice size must be positive");
+        if (p.dipole_cutoff < 1 || p.dipole_cutoff > 8)
+            return fail(c, SN_ERR_UNSUPPORTED, "DipoleCutOff must be in 1..8");
+        // every offset must wrap at most once (the reference's % wrap needs X > -dx too)
+        if (p.X <= p.dipole_cutoff || p.Y <= p.dipole_cutoff || (p.Z > 1 && p.Z <= p.dipole_cutoff))
+            return fail(c, SN_ERR_UNSUPPORTED, "each lattice axis (except Z == 1) must exceed DipoleCutOff");
+        if ((p.Z == 1) != (params[0].Z == 1))
+            return fail(c, SN_ERR_INVALID, "replicas of a context must all be 2D or all 3D");
         if (!(p.temperature >= 0.0)) return fail(c, SN_ERR_INVALID, "temperature must be >= 0");
         long long N = (long long)p.X * p.Y * p.Z;
         if (N > 0x7fffffffLL) return fail(c, SN_ERR_RANGE, "lattice larger than 2^31 sites");
@@ -229,7 +250,15 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         c->total_sites += N;
     }
     c->two_d = params[0].Z == 1;
-    if (!plan(c->geom, c->rc, c->tuning, c->sched)) return fail(c, SN_ERR_UNSUPPORTED, c->sched.error);
+    // A lattice that cannot be tiled still supports the gather entry points
+    // (site_energy, replay, energy, gradient); only sweeps are refused.
+    c->can_sweep = plan(c->geom, c->rc, c->tuning, c->sched);
+    if (!c->can_sweep) {
+        c->sweep_err = c->sched.error;
+        c->sched.shapes.clear();
+        c->sched.tiles.clear();
+        c->sched.coarse_start.assign(9, 0);
+    }
 
     // neighbour table: +-r pairs for the tile sweep, full offsets for gathers
     std::vector<Offset> pairs = neighbour_pairs(c->rc, c->two_d);
@@ -275,22 +304,31 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     for (const Tile &t : c->sched.tiles) td.push_back({t.replica, t.x0, t.y0, t.z0, t.shape, 0, 0, 0});
     if ((st = upload(c, &c->d_tiles, td))) return st;
     std::vector<ShapeDev> sd;
-    std::vector<int> cstart, csites, chunks;
+    std::vector<int> cstart, cri, chunks;
+    const int hz = c->two_d ? 0 : c->rc;
     for (const TileShape &s : c->sched.shapes) {
         ShapeDev d{};
         d.tx = s.tx; d.ty = s.ty; d.tz = s.tz; d.ncol = s.ncol;
         d.nchunks = (int)s.chunk_start.size() - 1;
         d.class_off = (int)cstart.size();
         d.chunk_off = (int)chunks.size();
-        d.sites_off = (int)csites.size();
+        d.sites_off = (int)cri.size();
         cstart.insert(cstart.end(), s.class_start.begin(), s.class_start.end());
-        csites.insert(csites.end(), s.class_sites.begin(), s.class_sites.end());
+        // tile-local index -> index into the LDS region (tile + halo)
+        const int RY = s.ty + 2 * c->rc, RZ = s.tz + 2 * hz;
+        for (int li : s.class_sites) {
+            int lz = li % s.tz, ly = (li / s.tz) % s.ty, lx = li / (s.tz * s.ty);
+            cri.push_back(((lx + c->rc) * RY + (ly + c->rc)) * RZ + (lz + hz));
+        }
         chunks.insert(chunks.end(), s.chunk_start.begin(), s.chunk_start.end());
+        c->max_tile_sites = std::max(c->max_tile_sites, s.tx * s.ty * s.tz);
+        c->max_cls = std::max(c->max_cls, s.ncol);
+        c->max_chunks = std::max(c->max_chunks, d.nchunks);
         sd.push_back(d);
     }
     if ((st = upload(c, &c->d_shapes, sd))) return st;
     if ((st = upload(c, &c->d_class_start, cstart))) return st;
-    if ((st = upload(c, &c->d_class_sites, csites))) return st;
+    if ((st = upload(c, &c->d_class_ri, cri))) return st;
     if ((st = upload(c, &c->d_chunk_col, chunks))) return st;
     if ((st = upload(c, &c->d_pairs, hp))) return st;
     if ((st = upload(c, &c->d_offs, ho))) return st;
@@ -301,11 +339,14 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     HIPCHK(c, hipMalloc((void **)&c->d_result, 64 * sizeof(double)));
     c->sweeps_done.assign(c->R, 0);
 
-    // LDS: 3 (or 4) region arrays + 4 x kBlock proposal slots
-    size_t mixed_lds = (size_t)(4 * c->sched.max_region + 4 * kBlock) * sizeof(double);
+    // LDS: 3 (or 4) region arrays + 4 x kBlock proposal slots + shape tables
+    size_t mixed_lds = sweep_lds(c, true);
     c->lds_bytes = mixed_lds;
-    if (mixed_lds > 160 * 1024)
-        return fail(c, SN_ERR_UNSUPPORTED, "tile + halo does not fit in LDS; use a smaller tile");
+    if (mixed_lds > 160 * 1024) {
+        c->can_sweep = false;
+        c->sweep_err = "tile + halo does not fit in LDS; use a smaller tile";
+    }
+    if (!c->can_sweep) return SN_OK;
     for (int m = 0; m < 2; m++) {
         SweepFn f = pick_sweep(G, NPL, m == 1);
         HIPCHK(c, hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -319,7 +360,7 @@ int sn_destroy(sn_ctx *c)
     if (!c) return SN_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph(c);
-    void *bufs[] = {c->d_lat, c->d_reps, c->d_tiles, c->d_shapes, c->d_class_start, c->d_class_sites,
+    void *bufs[] = {c->d_lat, c->d_reps, c->d_tiles, c->d_shapes, c->d_class_start, c->d_class_ri,
                     c->d_chunk_col, c->d_pairs, c->d_offs, c->d_sweep, c->d_counters, c->d_partials,
                     c->d_result};
     for (void *b : bufs)
@@ -479,16 +520,13 @@ static SweepArgs sweep_args(sn_ctx *c)
 {
     SweepArgs a;
     a.lat = c->d_lat; a.reps = c->d_reps; a.tiles = c->d_tiles; a.shapes = c->d_shapes;
-    a.class_start = c->d_class_start; a.class_sites = c->d_class_sites; a.chunk_col = c->d_chunk_col;
+    a.class_start = c->d_class_start; a.class_ri = c->d_class_ri; a.chunk_col = c->d_chunk_col;
     a.pairs = c->d_pairs; a.sweep_ctr = c->d_sweep; a.counters = c->d_counters;
     a.npairs = c->npairs; a.rc = c->rc; a.two_d = c->two_d ? 1 : 0;
     a.max_region = c->sched.max_region;
+    a.max_tile_sites = c->max_tile_sites;
+    a.max_cls = c->max_cls;
     return a;
-}
-
-static size_t sweep_lds(const sn_ctx *c, bool mixed)
-{
-    return (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * kBlock) * sizeof(double);
 }
 
 // Enqueue one sweep: one tile-sweep launch per non-empty coarse colour, then
@@ -540,6 +578,7 @@ static int build_graph(sn_ctx *c)
 int sn_mc_sweeps(sn_ctx *c, int32_t nsweeps)
 {
     if (!c) return SN_ERR_INVALID;
+    if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     if (nsweeps < 0) return fail(c, SN_ERR_INVALID, "negative sweep count");
     const bool use_graph = c->tuning.use_graph >= 0;  // 0 (auto) and 1 capture a graph
     if (use_graph) {
@@ -569,6 +608,7 @@ int sn_sync(sn_ctx *c)
 int sn_time_sweep_kernel(sn_ctx *c, int32_t nsweeps, double *mean_ms, int32_t *launches)
 {
     if (!c || nsweeps < 1 || !mean_ms) return fail(c, SN_ERR_INVALID, "bad arguments");
+    if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     std::vector<hipEvent_t> ev;
     int st = SN_OK;
     for (int i = 0; i < nsweeps && st == SN_OK; i++) st = enqueue_sweep(c, &ev);
@@ -746,6 +786,7 @@ int sn_schedule_info(sn_ctx *c, int32_t r, int64_t *n_sites, int32_t *n_phases)
 {
     int st = check_rep(c, r);
     if (st) return st;
+    if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     std::vector<int32_t> order, ps;
     sweep_order(c->sched, c->geom[r], r, order, ps);
     if (n_sites) *n_sites = (int64_t)order.size();
@@ -757,6 +798,7 @@ int sn_schedule_order(sn_ctx *c, int32_t r, int32_t *order, int32_t *phase_start
 {
     int st = check_rep(c, r);
     if (st) return st;
+    if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     std::vector<int32_t> o, ps;
     sweep_order(c->sched, c->geom[r], r, o, ps);
     if (order) std::memcpy(order, o.data(), o.size() * sizeof(int32_t));

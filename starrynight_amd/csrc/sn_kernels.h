@@ -18,14 +18,48 @@ struct SweepArgs {
     const RepDev *reps;
     const TileDev *tiles;
     const ShapeDev *shapes;
-    const int *class_start;   // per shape: ncol + 1 entries (site offsets into class_sites)
-    const int *class_sites;   // tile-local linear indices, grouped by fine colour
+    const int *class_start;   // per shape: ncol + 1 entries (offsets into class_ri)
+    const int *class_ri;      // per shape: LDS region index of each tile site, by fine colour
     const int *chunk_col;     // per shape: nchunks + 1 colour indices
     const PairDev *pairs;
     const unsigned *sweep_ctr;
     unsigned long long *counters;  // 2 per replica: accepted, rejected
     int npairs, rc, two_d, max_region;
+    int max_tile_sites, max_cls;   // sizes of the LDS copies of the shape tables
 };
+
+// DPP lane exchange of a double (two 32-bit moves); CTRL is a dpp_ctrl code.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double swz_xor16_f64(double v)
+{
+    // ds_swizzle bitmask mode: and 0x1F, or 0, xor 0x10 (within 32 lanes)
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);
+    hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over aligned groups of G lanes, result in every lane of the group:
+// butterfly of row_mirror (i <-> 15-i), row_half_mirror (i <-> 7-i) and
+// quad permutations xor 2, xor 1 (plus a swizzle xor 16 for G = 32).
+template <int G>
+__device__ __forceinline__ double group_sum_dpp(double v)
+{
+    if (G >= 32) v += swz_xor16_f64(v);
+    if (G >= 16) v += dpp_f64<0x140>(v);
+    if (G >= 8) v += dpp_f64<0x141>(v);
+    if (G >= 4) v += dpp_f64<0x4E>(v);
+    if (G >= 2) v += dpp_f64<0xB1>(v);
+    return v;
+}
 
 // One workgroup updates one tile: stage tile + halo in LDS, draw the
 // proposals of all tile sites, run the fine colours one after another (G lanes
@@ -52,18 +86,46 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
     double *sz = sy + a.max_region;
     double *sl = sz + a.max_region;                        // lengths (MIXED only)
     double *prop = MIXED ? sl + a.max_region : sl;         // 4 x kBlock: px, py, pz, threshold
+    int *s_ri = (int *)(prop + 4 * kBlock);                // region index of class sites
+    int *s_cls = s_ri + a.max_tile_sites;                  // class starts, then chunk colours
+    int *s_chunk = s_cls + a.max_cls + 1;
 
-    // ---- stage tile + halo (periodic wrap) ----
+    // ---- shape tables into LDS ----
+    const int ntile = sh.tx * sh.ty * sh.tz;
+    for (int i = tid; i < ntile; i += kBlock) s_ri[i] = a.class_ri[sh.sites_off + i];
+    for (int i = tid; i <= sh.ncol; i += kBlock) s_cls[i] = a.class_start[sh.class_off + i];
+    for (int i = tid; i <= sh.nchunks; i += kBlock) s_chunk[i] = a.chunk_col[sh.chunk_off + i];
+
+    // ---- stage tile + halo (periodic wrap), 4 loads in flight per lane ----
     const double4 *lat = a.lat + rep.base;
-    for (int i = tid; i < region; i += kBlock) {
-        int iz = i % RZ, q = i / RZ;
-        int iy = q % RY, ix = q / RY;
-        int gx = wrap_once(tile.x0 - hx + ix, rep.X);
-        int gy = wrap_once(tile.y0 - hy + iy, rep.Y);
-        int gz = wrap_once(tile.z0 - hz + iz, rep.Z);
-        double4 v = lat[((long long)gx * rep.Y + gy) * rep.Z + gz];
-        sx[i] = v.x; sy[i] = v.y; sz[i] = v.z;
-        if (MIXED) sl[i] = v.w;
+    {
+        // mixed-radix position of element tid, and the step of kBlock elements
+        int iz = tid % RZ, q = tid / RZ, iy = q % RY, ix = q / RY;
+        const int sz_ = kBlock % RZ, q2 = kBlock / RZ, sy_ = q2 % RY, sx_ = q2 / RY;
+        for (int base = tid; base < region; base += 4 * kBlock) {
+            double4 v[4];
+            int idx[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                idx[k] = base + k * kBlock;
+                if (idx[k] < region) {
+                    int gx = wrap_once(tile.x0 - hx + ix, rep.X);
+                    int gy = wrap_once(tile.y0 - hy + iy, rep.Y);
+                    int gz = wrap_once(tile.z0 - hz + iz, rep.Z);
+                    v[k] = lat[((long long)gx * rep.Y + gy) * rep.Z + gz];
+                }
+                iz += sz_; iy += sy_; ix += sx_;
+                if (iz >= RZ) { iz -= RZ; iy++; }
+                if (iy >= RY) { iy -= RY; ix++; }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (idx[k] < region) {
+                    sx[idx[k]] = v[k].x; sy[idx[k]] = v[k].y; sz[idx[k]] = v[k].z;
+                    if (MIXED) sl[idx[k]] = v[k].w;
+                }
+            }
+        }
     }
 
     // ---- this lane's share of the neighbour pairs, held in registers ----
@@ -88,21 +150,19 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
 
     const unsigned sweep = a.sweep_ctr[tile.replica];
     const double J = rep.J, K = rep.K, E0 = rep.E[0], E1 = rep.E[1], E2 = rep.E[2];
-    const int *cls = a.class_start + sh.class_off;
-    const int *sites = a.class_sites + sh.sites_off;
-    const int *chunks = a.chunk_col + sh.chunk_off;
+    const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
     unsigned nacc = 0, nrej = 0;
     __syncthreads();
 
     for (int ch = 0; ch < sh.nchunks; ch++) {
-        const int c0 = chunks[ch], c1 = chunks[ch + 1];
-        const int s0 = cls[c0], s1 = cls[c1];
+        const int c0 = s_chunk[ch], c1 = s_chunk[ch + 1];
+        const int s0 = s_cls[c0], s1 = s_cls[c1];
         // ---- proposals of this chunk's sites: one site per lane ----
         if (tid < s1 - s0) {
-            int li = sites[s0 + tid];
-            int lz = li % sh.tz, q = li / sh.tz;
-            int ly = q % sh.ty, lx = q / sh.ty;
-            unsigned gsite = (unsigned)(((tile.x0 + lx) * rep.Y + tile.y0 + ly) * rep.Z + tile.z0 + lz);
+            int ri = s_ri[s0 + tid];
+            int iz = ri % RZ, q = ri / RZ;
+            int iy = q % RY, ix = q / RY;
+            unsigned gsite = (unsigned)(tile_base + ((long long)(ix - hx) * rep.Y + (iy - hy)) * rep.Z + (iz - hz));
             double px, py, pz, u;
             counter_draw(rep.seed, (uint32_t)tile.replica, sweep, gsite, rep.constrain_to_x, px, py, pz, u);
             prop[tid] = px;
@@ -112,14 +172,11 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
         }
         __syncthreads();
         for (int c = c0; c < c1; c++) {
-            const int cs = cls[c], ce = cls[c + 1];
+            const int cs = s_cls[c], ce = s_cls[c + 1];
             for (int sb = cs; sb < ce; sb += ngroups) {
                 const int si = sb + group;
                 const bool active = si < ce;
-                const int li = sites[active ? si : cs];
-                int lz = li % sh.tz, q = li / sh.tz;
-                int ly = q % sh.ty, lx = q / sh.ty;
-                const int ri = ((lx + hx) * RY + (ly + hy)) * RZ + (lz + hz);
+                const int ri = s_ri[active ? si : cs];
 
                 double hx_ = 0.0, hy_ = 0.0, hz_ = 0.0;
                 double gx_ = 0.0, gy_ = 0.0, gz_ = 0.0;
@@ -143,8 +200,7 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
                     hz_ += w[m] * mz + ts * nz[m];
                     if (m == 0) { gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz; }
                 }
-                const int pi = si - s0;
-                const int pq = active ? pi : 0;
+                const int pq = active ? si - s0 : 0;
                 const double px = prop[pq], py = prop[kBlock + pq], pz = prop[2 * kBlock + pq];
                 const double thr = prop[3 * kBlock + pq];
                 const double ox = sx[ri], oy = sy[ri], oz = sz[ri];
@@ -152,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
                 const double coef = MIXED ? Li : Li * rep.L0;
                 const double dx = px - ox, dy = py - oy, dz = pz - oz;
                 double partial = coef * (dx * hx_ + dy * hy_ + dz * hz_) - J * (dx * gx_ + dy * gy_ + dz * gz_);
-                partial = group_sum<G>(partial);
+                partial = group_sum_dpp<G>(partial);
                 double dE = partial + (dx * E0 + dy * E1 + dz * E2);
                 if (K > 0.0) dE += -K * (fabs(px) - fabs(ox) + fabs(py) - fabs(oy));
                 const bool acc = dE < thr;
@@ -169,7 +225,6 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
 
     // ---- write the tile back ----
     double4 *out = a.lat + rep.base;
-    const int ntile = sh.tx * sh.ty * sh.tz;
     for (int i = tid; i < ntile; i += kBlock) {
         int lz = i % sh.tz, q = i / sh.tz;
         int ly = q % sh.ty, lx = q / sh.ty;
@@ -177,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
         double4 v;
         v.x = sx[ri]; v.y = sy[ri]; v.z = sz[ri];
         v.w = MIXED ? sl[ri] : rep.L0;
-        out[((long long)(tile.x0 + lx) * rep.Y + tile.y0 + ly) * rep.Z + tile.z0 + lz] = v;
+        out[tile_base + ((long long)lx * rep.Y + ly) * rep.Z + lz] = v;
     }
     // ---- counters: one atomic per wave ----
     unsigned long long ca = nacc, cr = nrej;

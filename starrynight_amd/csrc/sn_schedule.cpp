@@ -118,7 +118,7 @@ Sublattice make_sublattice(int rc, bool two_d, int kind)
     return make_sublattice(rc, two_d, 1);
 }
 
-bool split_axis(int L, int rc, int target, std::vector<int> &start, std::vector<int> &size)
+bool split_axis(int L, int rc, int target, int tmax, std::vector<int> &start, std::vector<int> &size)
 {
     start.clear(); size.clear();
     if (L == 1) { start.push_back(0); size.push_back(1); return true; }
@@ -128,7 +128,7 @@ bool split_axis(int L, int rc, int target, std::vector<int> &start, std::vector<
     for (int n = 2; n <= L; n += 2) {
         int lo = L / n, hi = (L + n - 1) / n;
         if (lo < rc) break;
-        if (hi + 2 * rc > L) continue;
+        if (hi + 2 * rc > L || hi > tmax) continue;
         double cost = std::fabs((double)L / n - target);
         if (cost < best_cost - 1e-9) { best_cost = cost; best_n = n; }
     }
@@ -143,7 +143,7 @@ bool split_axis(int L, int rc, int target, std::vector<int> &start, std::vector<
 }
 
 bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int target[3],
-                    int colouring_kind, int chunk, Schedule &out)
+                    int colouring_kind, int chunk, int max_region, Schedule &out)
 {
     out = Schedule();
     out.rc = rc;
@@ -154,6 +154,14 @@ bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int targ
         if ((g.Z == 1) != out.two_d) { out.error = "replicas mix 2D and 3D lattices"; return false; }
     out.lat = make_sublattice(rc, out.two_d, colouring_kind);
     int hz = out.two_d ? 0 : rc;
+    // largest tile edge whose tile + halo stays within max_region sites
+    int tmax = 1;
+    while (true) {
+        int e = tmax + 1 + 2 * rc;
+        long long reg = out.two_d ? (long long)e * e : (long long)e * e * e;
+        if (reg > max_region) break;
+        tmax++;
+    }
 
     std::map<std::tuple<int, int, int>, int> shape_id;
     std::vector<std::vector<Tile>> by_colour(8);
@@ -161,11 +169,12 @@ bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int targ
         const ReplicaGeom &g = reps[r];
         if (g.X < 1 || g.Y < 1 || g.Z < 1) { out.error = "lattice size must be positive"; return false; }
         std::vector<int> sx, lx, sy, ly, sz, lz;
-        if (!split_axis(g.X, rc, target[0], sx, lx) || !split_axis(g.Y, rc, target[1], sy, ly) ||
-            !split_axis(g.Z, rc, target[2], sz, lz)) {
+        if (!split_axis(g.X, rc, target[0], tmax, sx, lx) || !split_axis(g.Y, rc, target[1], tmax, sy, ly) ||
+            !split_axis(g.Z, rc, target[2], tmax, sz, lz)) {
             std::ostringstream m;
             m << "lattice " << g.X << "x" << g.Y << "x" << g.Z << " cannot be tiled for DipoleCutOff="
-              << rc << " (each axis needs length >= 3*DipoleCutOff, or Z == 1)";
+              << rc << " (each axis needs an even number of tiles of edge >= DipoleCutOff with tile + "
+              << "2*DipoleCutOff <= length and tile <= " << tmax << ", or Z == 1)";
             out.error = m.str();
             return false;
         }

@@ -39,11 +39,11 @@ struct Sublattice {
 // 3 body-centred (4,0,0),(0,4,0),(2,2,2) (valid for rc <= 3).
 Sublattice make_sublattice(int rc, bool two_d, int kind);
 
-// Split an axis of length L into an even number of tiles, each of size >= rc,
-// near `target`, with tile + 2*rc <= L (so a tile's halo never wraps onto
-// itself).  L == 1 (the z axis of a 2D lattice) gives one tile of size 1.
-// Returns false when no such split exists.
-bool split_axis(int L, int rc, int target, std::vector<int> &start, std::vector<int> &size);
+// Split an axis of length L into an even number of tiles, each of size >= rc
+// and <= tmax, near `target`, with tile + 2*rc <= L (so a tile's halo never
+// wraps onto itself).  L == 1 (the z axis of a 2D lattice) gives one tile of
+// size 1.  Returns false when no such split exists.
+bool split_axis(int L, int rc, int target, int tmax, std::vector<int> &start, std::vector<int> &size);
 
 struct TileShape {
     int tx, ty, tz;             // tile size
@@ -77,9 +77,10 @@ struct Schedule {
 };
 
 // Build the schedule for all replicas.  target[3] = tile edge aimed for per
-// axis; chunk = block size (sites whose proposals are drawn at once).
+// axis; chunk = block size (sites whose proposals are drawn at once);
+// max_region = largest tile + halo (sites) the LDS budget allows.
 bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int target[3],
-                    int colouring_kind, int chunk, Schedule &out);
+                    int colouring_kind, int chunk, int max_region, Schedule &out);
 
 // Visit order of replica r for one sweep and its phase boundaries
 // (phase = one (coarse colour, fine colour) pair across all tiles).

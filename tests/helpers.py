@@ -28,5 +28,5 @@ def beta_of_T(T):
     return 300.0 / T
 
 
-def print_case(name, **kw):
-    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+def print_case(label, **kw):
+    print(f"[case] {label} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)

@@ -134,3 +134,16 @@ def test_device_random_init(sn):
     frac = (lat[:, 3] == 1.0).mean()
     assert abs(frac - 0.5) < 0.02 and set(np.unique(lat[:, 3])) == {0.0, 1.0}
     assert np.abs(lat[:, :3].mean(0)).max() < 0.02
+
+
+def test_untileable_lattice_refuses_sweeps_but_gathers_work(sn, oracle):
+    rp = sn.ReplicaParams(9, 9, 9, T=300)
+    lat = rand_lattice(rp.N, np.random.default_rng(9))
+    with sn.StarryNight(rp) as ctx:
+        ctx.set_lattice(lat)
+        with pytest.raises(sn.SnError) as e:
+            ctx.MC_sweeps(1)
+        assert e.value.status == 5
+        dE = ctx.site_energy_batch(np.arange(10, dtype=np.int32), lat[10:20, :3])
+    want, _ = oracle_model(oracle, rp).site_energy_batch(lat, np.arange(10), lat[10:20, :3])
+    assert np.abs(dE - want).max() < 1e-12 * 100
