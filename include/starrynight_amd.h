@@ -57,6 +57,8 @@ typedef struct sn_tuning {
     int32_t lanes_per_site;   /* lanes that share one site's neighbour sum: 2..64 */
     int32_t fine_colouring;   /* 0 auto, 1 cubic (rc+1)^3, 2 min-index sublattice, 3 BCC (rc=3) */
     int32_t use_graph;        /* 0 or 1: replay sweeps from a captured hipGraph; -1: direct launches */
+    int32_t kernel;           /* 0 auto, 1 field-update tile kernel, 2 per-phase gather tile kernel
+                                 (both visit sites in the same order with the same draws) */
 } sn_tuning;
 
 /* ---- lifecycle ---------------------------------------------------------- */
@@ -88,6 +90,18 @@ int sn_lattice_device_ptr(sn_ctx *ctx, int32_t replica, void **dev_ptr);
  * not need the MT19937 initial state. */
 int sn_init_lattice_random(sn_ctx *ctx, int32_t replica, int32_t n_species,
                            const double *lengths, const double *prevalence);
+
+/* Reference-compatible initial state, on the host (no context, no device):
+ * init_genrand(seed), the InitialLattice pattern ("random", "ferroelectric",
+ * "buckled", "antiferro_wall", "ferro_wall", "antiferro_slip", "spectrum",
+ * "slab_delete"; any other name is "random"), then solid_solution with the
+ * Dipoles / Prevalence table -- the float values the reference holds after
+ * src/starrynight-main.c:172-205 (src/starrynight-lattice.c:25-171), written as
+ * double[N][4].  The reference seeds with 0xDEADBEEF + T.  SN_ERR_INVALID when
+ * Prevalence sums below the drawn sample (the reference reads past its table). */
+int sn_reference_initial_lattice(int32_t X, int32_t Y, int32_t Z, const char *initial_lattice,
+                                 uint32_t seed, int32_t n_species, const double *lengths,
+                                 const double *prevalence, double *xyzl);
 
 /* ---- parameters --------------------------------------------------------- */
 

@@ -24,7 +24,7 @@ EXPORTS = [
     "sn_site_energy_batch", "sn_site_energy_batch_device", "sn_mc_sweeps", "sn_sync",
     "sn_mc_replay", "sn_counters", "sn_reset_counters", "sn_energy", "sn_gradient", "sn_moment",
     "sn_schedule_info", "sn_schedule_order", "sn_stream", "sn_sweep_config",
-    "sn_time_sweep_kernel", "sn_plan_schedule",
+    "sn_time_sweep_kernel", "sn_plan_schedule", "sn_reference_initial_lattice",
 ]
 
 
@@ -42,7 +42,7 @@ class Params(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_int32 * 3), ("lanes_per_site", C.c_int32),
-                ("fine_colouring", C.c_int32), ("use_graph", C.c_int32)]
+                ("fine_colouring", C.c_int32), ("use_graph", C.c_int32), ("kernel", C.c_int32)]
 
 
 _lib = None
@@ -82,6 +82,8 @@ def lib():
         L.sn_stream.argtypes = [vp, P(vp)]
         L.sn_sweep_config.argtypes = [vp, ip, ip, ip, ip]
         L.sn_time_sweep_kernel.argtypes = [vp, C.c_int32, dp, ip]
+        L.sn_reference_initial_lattice.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_char_p, C.c_uint32,
+                                                   C.c_int32, dp, dp, dp]
         L.sn_plan_schedule.argtypes = [P(Params), C.c_int32, P(Tuning), C.c_int32, P(C.c_int64), ip,
                                        ip, ip]
         _lib = L

@@ -21,11 +21,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SN_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off (no FP32 math on the hot path anyway)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
+
+SOURCES = ["sn_api.hip", "sn_schedule.cpp", "sn_lattice_init.cpp", "sn_gather_kernels.hip"] + \
+    [f"sn_field_kernels_rc{r}.hip" for r in range(1, 6)]
 
 
 def _run(cmd):
@@ -33,10 +36,23 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
-def build_lib(extra=()):
-    srcs = [os.path.join(CSRC, "sn_api.hip"), os.path.join(CSRC, "sn_schedule.cpp")]
+def build_lib(extra=(), jobs=None):
+    """Compile every translation unit (in parallel) into _build/, link the .so."""
+    import concurrent.futures as cf
+    bdir = os.path.join(PKG, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    cmds = []
+    for s in SOURCES:
+        o = os.path.join(bdir, os.path.splitext(s)[0] + ".o")
+        objs.append(o)
+        cmds.append([HIPCC, *HIP_FLAGS, *extra, "-c", "-o", o, os.path.join(CSRC, s)])
+    jobs = jobs or min(len(cmds), os.cpu_count() or 4, 16)
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, *extra, "-o", tmp, *srcs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, LIB)
 
 

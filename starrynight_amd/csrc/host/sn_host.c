@@ -1,0 +1,247 @@
+/*
+ * sn_host.c -- the C driver: reads starrynight.cfg and runs the Metropolis
+ * sweeps on the GPU through the C ABI.  It plays the role of the reference's
+ * main() (src/starrynight-main.c:125-277): same config keys, same argv
+ * overrides (argv[1] = T, argv[2] = CageStrain, :142-151), same seed
+ * 0xDEADBEEF + T (:172), same equilibration / production loop with the
+ * "MC Moves (per second): %f MHz" line (:219-259) and the final
+ * ACCEPT / REJECT ratio (:273).  The analysis and image writers of the
+ * reference are not part of this driver.
+ *
+ * Keys added for the GPU build (a libconfig reader ignores unknown keys):
+ *   TemperatureSchedule = [ ... ]   one replica per temperature, run batched
+ *   Replicas = n                    copies of each temperature (seeds differ)
+ *   EfieldRamp = { axis = "z"; from = -0.1; to = 0.1; steps = 41; sweeps = 500; }
+ *   Seed = n                        RNG key (default 0xDEADBEEF + T)
+ *   Tile = [8, 8, 8]                tile edge the schedule aims for
+ *   SaveLattice = "file"            final lattice of replica 0, double[N][4]
+ *
+ * usage: sn_host [-c config] [T [CageStrain]]
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "sn_config.h"
+#include "starrynight_amd.h"
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+#define CHECK(ctx, call)                                                                   \
+    do {                                                                                   \
+        int st_ = (call);                                                                  \
+        if (st_ != SN_OK) {                                                                \
+            fprintf(stderr, "%s failed: %s (%s)\n", #call, sn_status_string(st_),         \
+                    (ctx) ? sn_last_error(ctx) : "");                                      \
+            return 1;                                                                      \
+        }                                                                                  \
+    } while (0)
+
+typedef struct {
+    int T, X, Y, Z, rc, constrain_x, eqm, mega;
+    double E[3], K, J, moves;
+    int nspec;
+    double len[10], prev[10];
+    const char *initial;
+} run_cfg;
+
+/* load_config (src/starrynight-config.c:98-182): the reference's defaults
+ * (:12-93), then whatever keys the file sets, with libconfig's typing. */
+static int load(const char *path, snc_config *cf, run_cfg *r)
+{
+    r->T = 0; r->X = 20; r->Y = 20; r->Z = 20; r->rc = 3; r->constrain_x = 0;
+    r->eqm = 10; r->mega = 400; r->moves = 1.0;
+    r->E[0] = r->E[1] = r->E[2] = 0.0; r->K = 1.0; r->J = 1.0;
+    r->nspec = 1; r->len[0] = 1.0; r->prev[0] = 1.0;
+    r->initial = "random";
+    if (snc_read_file(cf, path)) {
+        fprintf(stderr, "%s:%d - %s\n", path, cf->error_line, cf->error);
+        return -1;
+    }
+    double t;
+    snc_lookup_int(cf, "T", &r->T);
+    snc_lookup_int(cf, "X", &r->X);
+    snc_lookup_int(cf, "Y", &r->Y);
+    snc_lookup_int(cf, "Z", &r->Z);
+    /* Efield is narrowed to float (src/starrynight-config.c:132-134) */
+    if (snc_lookup_float(cf, "Efield.x", &t)) r->E[0] = (float)t;
+    if (snc_lookup_float(cf, "Efield.y", &t)) r->E[1] = (float)t;
+    if (snc_lookup_float(cf, "Efield.z", &t)) r->E[2] = (float)t;
+    snc_lookup_float(cf, "K", &r->K);
+    snc_lookup_float(cf, "CageStrain", &r->J);
+    const snc_node *d = snc_lookup(cf, "Dipoles"), *p = snc_lookup(cf, "Prevalence");
+    if (d && p) {
+        int n = snc_length(p);
+        if (n < 1 || n > 10 || snc_length(d) < n) {
+            fprintf(stderr, "Dipoles / Prevalence must list 1..10 species\n");
+            return -1;
+        }
+        r->nspec = n;
+        for (int i = 0; i < n; i++) {
+            r->len[i] = (float)snc_float_elem(d, i);
+            r->prev[i] = (float)snc_float_elem(p, i);
+        }
+    }
+    snc_lookup_bool(cf, "ConstrainToX", &r->constrain_x);
+    snc_lookup_int(cf, "DipoleCutOff", &r->rc);
+    snc_lookup_string(cf, "InitialLattice", &r->initial);
+    snc_lookup_int(cf, "MCEqmSteps", &r->eqm);
+    snc_lookup_int(cf, "MCMegaSteps", &r->mega);
+    snc_lookup_float(cf, "MCMoves", &r->moves);
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    const char *path = "starrynight.cfg";
+    int ai = 1;
+    if (argc > 2 && strcmp(argv[1], "-c") == 0) { path = argv[2]; ai = 3; }
+    snc_config cf;
+    run_cfg r;
+    fprintf(stderr, "Starry Night (MI355X) - Monte Carlo brushstrokes.\n");
+    if (load(path, &cf, &r)) return 1;
+    if (argc > ai) { sscanf(argv[ai], "%d", &r.T); fprintf(stderr, "Command line temperature: T = %d\n", r.T); }
+    if (argc > ai + 1) {
+        sscanf(argv[ai + 1], "%lf", &r.J);
+        fprintf(stderr, "Command Line CageStrain: CageStrain = %lf\n", r.J);
+    }
+
+    /* replicas: one per scheduled temperature (or the single T), times Replicas */
+    int nT = 1, copies = 1;
+    double Ts[1024];
+    Ts[0] = r.T;
+    const snc_node *sched = snc_lookup(&cf, "TemperatureSchedule");
+    if (sched && snc_length(sched) > 0) {
+        nT = snc_length(sched) < 1024 ? snc_length(sched) : 1024;
+        for (int i = 0; i < nT; i++) {
+            const snc_node *e = sched->kids[i];
+            Ts[i] = e->type == SNC_INT ? (double)e->ival : e->fval;
+        }
+    }
+    snc_lookup_int(&cf, "Replicas", &copies);
+    if (copies < 1) copies = 1;
+    int R = nT * copies;
+    int64_t seed_base = (int64_t)(uint32_t)(0xDEADBEEFu + (uint32_t)r.T);
+    int have_seed = snc_lookup_int64(&cf, "Seed", &seed_base);
+
+    sn_params *ps = (sn_params *)calloc(R, sizeof(sn_params));
+    if (!ps) return 1;
+    for (int i = 0; i < R; i++) {
+        double Ti = Ts[i % nT];
+        ps[i].X = r.X; ps[i].Y = r.Y; ps[i].Z = r.Z; ps[i].dipole_cutoff = r.rc;
+        ps[i].cage_strain = r.J;
+        ps[i].efield[0] = r.E[0]; ps[i].efield[1] = r.E[1]; ps[i].efield[2] = r.E[2];
+        ps[i].K = r.K; ps[i].constrain_to_x = r.constrain_x; ps[i].temperature = Ti;
+        ps[i].seed = have_seed ? (uint64_t)seed_base + (uint64_t)i
+                               : (uint64_t)(uint32_t)(0xDEADBEEFu + (uint32_t)(int)Ti) + ((uint64_t)(i / nT) << 32);
+    }
+    sn_tuning tun;
+    memset(&tun, 0, sizeof(tun));
+    const snc_node *tile = snc_lookup(&cf, "Tile");
+    if (tile && snc_length(tile) == 3)
+        for (int k = 0; k < 3; k++) tun.tile[k] = tile->kids[k]->type == SNC_INT ? (int)tile->kids[k]->ival : 0;
+
+    fprintf(stderr, "Lattice %dx%dx%d, DipoleCutOff %d, CageStrain %f, Efield (%f %f %f), K %f, %d replica(s)\n",
+            r.X, r.Y, r.Z, r.rc, r.J, r.E[0], r.E[1], r.E[2], r.K, R);
+    sn_ctx *ctx = NULL;
+    int st = sn_create(ps, R, &tun, &ctx);
+    if (st != SN_OK) {
+        fprintf(stderr, "sn_create failed: %s (%s)\n", sn_status_string(st), ctx ? sn_last_error(ctx) : "");
+        return 1;
+    }
+    long long N = (long long)r.X * r.Y * r.Z;
+    double *lat = (double *)malloc(sizeof(double) * 4 * N);
+    if (!lat) return 1;
+    for (int i = 0; i < R; i++) {
+        /* Mersenne Twister seeded 0xDEADBEEF + T, as the reference (:172-176) */
+        uint32_t mt_seed = (uint32_t)(0xDEADBEEFu + (uint32_t)(int)Ts[i % nT]);
+        CHECK(ctx, sn_reference_initial_lattice(r.X, r.Y, r.Z, r.initial, mt_seed, r.nspec, r.len, r.prev, lat));
+        CHECK(ctx, sn_set_lattice(ctx, i, lat));
+    }
+    int sweeps = (int)lround(r.moves);
+    if (fabs(r.moves - sweeps) > 1e-9)
+        fprintf(stderr, "MCMoves = %g: the GPU runs whole sweeps, using %d per megastep\n", r.moves, sweeps);
+    fprintf(stderr, "'.' is %e MC moves attempted.\n", (double)N * sweeps);
+
+    fprintf(stderr, "Equilibriation MC moves... %e\n", (double)N * sweeps * r.eqm);
+    for (int i = 0; i < r.eqm; i++) CHECK(ctx, sn_mc_sweeps(ctx, sweeps));
+    CHECK(ctx, sn_sync(ctx));
+
+    /* optional field ramp (hysteresis): steps of `sweeps_per` sweeps each */
+    const snc_node *ramp = snc_lookup(&cf, "EfieldRamp");
+    if (ramp) {
+        const char *axis = "z";
+        double from = -0.1, to = 0.1;
+        int steps = 41, per = 500;
+        snc_lookup_string(&cf, "EfieldRamp.axis", &axis);
+        snc_lookup_float(&cf, "EfieldRamp.from", &from);
+        snc_lookup_float(&cf, "EfieldRamp.to", &to);
+        snc_lookup_int(&cf, "EfieldRamp.steps", &steps);
+        snc_lookup_int(&cf, "EfieldRamp.sweeps", &per);
+        int ax = axis[0] == 'x' ? 0 : axis[0] == 'y' ? 1 : 2;
+        printf("# step E_%c  mean_P_%c_over_replicas\n", "xyz"[ax], "xyz"[ax]);
+        for (int s = 0; s < steps; s++) {
+            double Ev = steps > 1 ? from + (to - from) * s / (steps - 1) : from;
+            for (int i = 0; i < R; i++) {
+                double E[3] = {r.E[0], r.E[1], r.E[2]};
+                E[ax] = Ev;
+                CHECK(ctx, sn_set_params(ctx, i, Ts[i % nT], E, r.J));
+            }
+            CHECK(ctx, sn_mc_sweeps(ctx, per));
+            double Pm = 0.0;
+            for (int i = 0; i < R; i++) {
+                double M[3];
+                CHECK(ctx, sn_moment(ctx, i, M));
+                Pm += M[ax] / (double)N;
+            }
+            printf("%d %f %f\n", s, Ev, Pm / R);
+            fflush(stdout);
+        }
+    }
+
+    /* production: MCMegaSteps x MC_moves, timed around the moves only */
+    printf("# megastep replica T energy_per_site |P| P_x\n");
+    for (int m = 0; m < r.mega; m++) {
+        double t0 = now_s();
+        CHECK(ctx, sn_mc_sweeps(ctx, sweeps));
+        CHECK(ctx, sn_sync(ctx));
+        double t1 = now_s();
+        for (int i = 0; i < R; i++) {
+            double M[3], E;
+            CHECK(ctx, sn_moment(ctx, i, M));
+            CHECK(ctx, sn_energy(ctx, i, &E, NULL));
+            printf("%d %d %g %f %f %f\n", m, i, Ts[i % nT], E / N,
+                   sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]) / N, M[0] / N);
+        }
+        fflush(stdout);
+        fprintf(stderr, "MC Moves (per second): %f MHz\n", 1e-6 * (double)N * sweeps * R / (t1 - t0));
+    }
+    uint64_t A = 0, Rj = 0;
+    for (int i = 0; i < R; i++) {
+        uint64_t a, rj;
+        CHECK(ctx, sn_counters(ctx, i, &a, &rj, NULL));
+        A += a; Rj += rj;
+    }
+    fprintf(stderr, "Monte Carlo moves - ACCEPT: %llu REJECT: %llu ratio: %f\n", (unsigned long long)A,
+            (unsigned long long)Rj, (float)A / (float)(A + Rj));
+    const char *save = NULL;
+    if (snc_lookup_string(&cf, "SaveLattice", &save)) {
+        CHECK(ctx, sn_get_lattice(ctx, 0, lat));
+        FILE *f = fopen(save, "wb");
+        if (!f || fwrite(lat, sizeof(double) * 4, N, f) != (size_t)N) { fprintf(stderr, "cannot write %s\n", save); return 1; }
+        fclose(f);
+    }
+    free(lat);
+    free(ps);
+    sn_destroy(ctx);
+    snc_destroy(&cf);
+    return 0;
+}

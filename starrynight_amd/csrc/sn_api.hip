@@ -9,7 +9,8 @@
 #include <vector>
 
 #include "../../include/starrynight_amd.h"
-#include "sn_kernels.h"
+#include "sn_aux_kernels.h"
+#include "sn_dispatch.h"
 #include "sn_schedule.h"
 
 using namespace sn;
@@ -22,8 +23,6 @@ struct RepHost {
     int mixed;
     double L0;
 };
-
-typedef void (*SweepFn)(SweepArgs, int, int);
 
 }  // namespace
 
@@ -46,12 +45,16 @@ struct sn_ctx {
     TileDev *d_tiles = nullptr;
     ShapeDev *d_shapes = nullptr;
     int *d_class_start = nullptr, *d_class_ri = nullptr, *d_chunk_col = nullptr;
+    int *d_class_tile = nullptr, *d_tile_coord = nullptr;
     PairDev *d_pairs = nullptr, *d_offs = nullptr;
     unsigned *d_sweep = nullptr;
-    unsigned long long *d_counters = nullptr;
+    unsigned long long *d_counters = nullptr;   // 2 per tile (accepted, rejected)
+    std::vector<unsigned long long> replay_counts;  // 2 per replica, from sn_mc_replay
+    bool use_field = true;
     double *d_partials = nullptr, *d_result = nullptr;
     size_t partial_cap = 0;
     int max_tile_sites = 0, max_cls = 0, max_chunks = 0;
+    int field_region = 0;   // largest staged region of the field kernel (z rows padded to odd)
     bool can_sweep = false;
     std::string sweep_err;
     std::vector<unsigned long long> sweeps_done;
@@ -74,21 +77,6 @@ static int fail(sn_ctx *c, int code, const std::string &msg)
         if (e_ != hipSuccess)                                                               \
             return fail(ctx, SN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
-
-// ---- sweep kernel instantiations ------------------------------------------
-#define SN_SWEEP_LIST(X) \
-    X(4, 4) X(4, 8) X(4, 16) X(8, 2) X(8, 4) X(8, 8) X(8, 16) X(16, 2) X(16, 4) X(16, 8) X(16, 16) \
-    X(32, 2) X(32, 4) X(32, 8)
-
-static SweepFn pick_sweep(int G, int NPL, bool mixed)
-{
-#define SN_CASE(g, n)                                                                  \
-    if (G == g && NPL == n) return mixed ? (SweepFn)tile_sweep_kernel<g, n, true>     \
-                                         : (SweepFn)tile_sweep_kernel<g, n, false>;
-    SN_SWEEP_LIST(SN_CASE)
-#undef SN_CASE
-    return nullptr;
-}
 
 static bool any_mixed(const sn_ctx *c)
 {
@@ -158,6 +146,9 @@ static int ensure_partials(sn_ctx *c, size_t n)
 // Largest tile + halo (sites) staged in LDS: 4 doubles per site (mixed lengths)
 // plus the proposal slots and shape tables stay under the 160 KB of a CU.
 constexpr int kMaxRegion = 4096;
+// Largest tile: the field kernel gives each lane at most 2 items of 2 sites
+// (3D) or 1 site (2D).
+constexpr int kMaxTile3D = 4 * kBlock, kMaxTile2D = 2 * kBlock;
 
 static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &t, Schedule &s)
 {
@@ -166,14 +157,32 @@ static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &
     bool two_d = geom[0].Z == 1;
     int kind = t.fine_colouring;
     if (kind == 0) kind = (!two_d && rc == 3) ? 3 : 2;
-    return build_schedule(geom, rc, target, kind, kBlock, kMaxRegion, s);
+    return build_schedule(geom, rc, target, kind, kBlock, kMaxRegion, two_d ? kMaxTile2D : kMaxTile3D, s);
 }
 
+// dynamic LDS bytes of the per-phase gather kernel
 static size_t sweep_lds(const sn_ctx *c, bool mixed)
 {
     size_t dbl = (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * kBlock) * sizeof(double);
     size_t ints = (size_t)(c->max_tile_sites + c->max_cls + 1 + c->max_chunks + 1) * sizeof(int);
     return dbl + ints;
+}
+
+// LDS doubles of the field kernel's shared area: the staged region (padded)
+// first, then the same space as class-ordered compact arrays.
+static int field_region_doubles(const sn_ctx *c) { return c->field_region + 2; }
+
+static int field_phaseA(const sn_ctx *c, bool mixed)
+{
+    int region = (mixed ? 4 : 3) * field_region_doubles(c);
+    int compact = (mixed ? 11 : 10) * c->max_tile_sites;
+    return std::max(region, compact);
+}
+
+static size_t field_lds(const sn_ctx *c, bool mixed)
+{
+    return (size_t)field_phaseA(c, mixed) * sizeof(double) +
+           (size_t)(2 * c->max_tile_sites + c->max_cls + 1) * sizeof(int);
 }
 
 extern "C" {
@@ -288,9 +297,10 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     int need = (c->npairs + G - 1) / G;
     int NPL = 2;
     while (NPL < need) NPL <<= 1;
-    if (!pick_sweep(G, NPL, false))
-        return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
     c->G = G; c->NPL = NPL;
+    const bool field_ok = pick_field(c->rc, c->two_d, false) != nullptr;
+    if ((c->tuning.kernel == 2 || !field_ok) && !pick_sweep(G, NPL, false))
+        return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
 
     HIPCHK(c, hipGetDevice(&c->device));
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -304,7 +314,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     for (const Tile &t : c->sched.tiles) td.push_back({t.replica, t.x0, t.y0, t.z0, t.shape, 0, 0, 0});
     if ((st = upload(c, &c->d_tiles, td))) return st;
     std::vector<ShapeDev> sd;
-    std::vector<int> cstart, cri, chunks;
+    std::vector<int> cstart, cri, chunks, ctile, tcoord;
     const int hz = c->two_d ? 0 : c->rc;
     for (const TileShape &s : c->sched.shapes) {
         ShapeDev d{};
@@ -314,14 +324,24 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         d.chunk_off = (int)chunks.size();
         d.sites_off = (int)cri.size();
         cstart.insert(cstart.end(), s.class_start.begin(), s.class_start.end());
-        // tile-local index -> index into the LDS region (tile + halo)
+        // by fine colour: region index (tile + halo) and tile-local index of
+        // each site; by tile index: packed local coordinates
         const int RY = s.ty + 2 * c->rc, RZ = s.tz + 2 * hz;
-        for (int li : s.class_sites) {
+        const int ntile = s.tx * s.ty * s.tz;
+        for (size_t k = 0; k < s.class_sites.size(); k++) {
+            int li = s.class_sites[k];
             int lz = li % s.tz, ly = (li / s.tz) % s.ty, lx = li / (s.tz * s.ty);
             cri.push_back(((lx + c->rc) * RY + (ly + c->rc)) * RZ + (lz + hz));
+            ctile.push_back(li);
+        }
+        for (int li = 0; li < ntile; li++) {
+            int lz = li % s.tz, ly = (li / s.tz) % s.ty, lx = li / (s.tz * s.ty);
+            tcoord.push_back(lx | (ly << 10) | (lz << 20));
         }
         chunks.insert(chunks.end(), s.chunk_start.begin(), s.chunk_start.end());
         c->max_tile_sites = std::max(c->max_tile_sites, s.tx * s.ty * s.tz);
+        const int RZp = c->two_d ? RZ : (RZ | 1);
+        c->field_region = std::max(c->field_region, (s.tx + 2 * c->rc) * RY * RZp);
         c->max_cls = std::max(c->max_cls, s.ncol);
         c->max_chunks = std::max(c->max_chunks, d.nchunks);
         sd.push_back(d);
@@ -329,28 +349,35 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if ((st = upload(c, &c->d_shapes, sd))) return st;
     if ((st = upload(c, &c->d_class_start, cstart))) return st;
     if ((st = upload(c, &c->d_class_ri, cri))) return st;
+    if ((st = upload(c, &c->d_class_tile, ctile))) return st;
+    if ((st = upload(c, &c->d_tile_coord, tcoord))) return st;
     if ((st = upload(c, &c->d_chunk_col, chunks))) return st;
     if ((st = upload(c, &c->d_pairs, hp))) return st;
     if ((st = upload(c, &c->d_offs, ho))) return st;
     std::vector<unsigned> zeros_u(c->R, 0u);
     if ((st = upload(c, &c->d_sweep, zeros_u))) return st;
-    std::vector<unsigned long long> zeros_c(2 * c->R, 0ull);
+    std::vector<unsigned long long> zeros_c(2 * std::max<size_t>(c->sched.tiles.size(), 1), 0ull);
     if ((st = upload(c, &c->d_counters, zeros_c))) return st;
     HIPCHK(c, hipMalloc((void **)&c->d_result, 64 * sizeof(double)));
     c->sweeps_done.assign(c->R, 0);
+    c->replay_counts.assign(2 * c->R, 0ull);
 
-    // LDS: 3 (or 4) region arrays + 4 x kBlock proposal slots + shape tables
-    size_t mixed_lds = sweep_lds(c, true);
-    c->lds_bytes = mixed_lds;
-    if (mixed_lds > 160 * 1024) {
+    // kernel choice: the field-update kernel where it is instantiated
+    c->use_field = c->tuning.kernel != 2 && pick_field(c->rc, c->two_d, false) != nullptr;
+    if (c->tuning.kernel == 1 && !c->use_field)
+        return fail(c, SN_ERR_UNSUPPORTED, "field-update kernel needs DipoleCutOff in 1..5");
+    size_t need_lds = c->use_field ? std::max(field_lds(c, false), field_lds(c, true))
+                                   : sweep_lds(c, true);
+    c->lds_bytes = need_lds;
+    if (need_lds > 160 * 1024) {
         c->can_sweep = false;
         c->sweep_err = "tile + halo does not fit in LDS; use a smaller tile";
     }
     if (!c->can_sweep) return SN_OK;
     for (int m = 0; m < 2; m++) {
-        SweepFn f = pick_sweep(G, NPL, m == 1);
-        HIPCHK(c, hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)mixed_lds));
+        const void *f = c->use_field ? (const void *)pick_field(c->rc, c->two_d, m == 1)
+                                     : (const void *)pick_sweep(G, NPL, m == 1);
+        HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need_lds));
     }
     return SN_OK;
 }
@@ -361,8 +388,8 @@ int sn_destroy(sn_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph(c);
     void *bufs[] = {c->d_lat, c->d_reps, c->d_tiles, c->d_shapes, c->d_class_start, c->d_class_ri,
-                    c->d_chunk_col, c->d_pairs, c->d_offs, c->d_sweep, c->d_counters, c->d_partials,
-                    c->d_result};
+                    c->d_class_tile, c->d_tile_coord, c->d_chunk_col, c->d_pairs, c->d_offs, c->d_sweep,
+                    c->d_counters, c->d_partials, c->d_result};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -531,29 +558,43 @@ static SweepArgs sweep_args(sn_ctx *c)
 
 // Enqueue one sweep: one tile-sweep launch per non-empty coarse colour, then
 // the sweep-counter bump.  events (optional) bracket each tile-sweep launch.
+static int launch_coarse(sn_ctx *c, bool mixed, int first, int n)
+{
+    if (c->use_field) {
+        FieldArgs fa;
+        fa.s = sweep_args(c);
+        fa.class_tile = c->d_class_tile;
+        fa.tile_coord = c->d_tile_coord;
+        fa.tile_counts = c->d_counters;
+        fa.phaseA_doubles = field_phaseA(c, mixed);
+        fa.region_doubles = field_region_doubles(c);
+        hipLaunchKernelGGL(pick_field(c->rc, c->two_d, mixed), dim3(n), dim3(kBlock), field_lds(c, mixed),
+                           c->stream, fa, first, n);
+    } else {
+        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed), dim3(n), dim3(kBlock), sweep_lds(c, mixed),
+                           c->stream, sweep_args(c), first, n);
+    }
+    HIPCHK(c, hipGetLastError());
+    return SN_OK;
+}
+
 static int enqueue_sweep(sn_ctx *c, std::vector<hipEvent_t> *events)
 {
     const bool mixed = any_mixed(c);
-    SweepFn f = pick_sweep(c->G, c->NPL, mixed);
-    SweepArgs a = sweep_args(c);
-    size_t lds = sweep_lds(c, mixed);
     for (int cc = 0; cc < 8; cc++) {
         int first = c->sched.coarse_start[cc], n = c->sched.coarse_start[cc + 1] - first;
         if (n <= 0) continue;
+        int st;
         if (events) {
             hipEvent_t e0, e1;
             HIPCHK(c, hipEventCreate(&e0));
             HIPCHK(c, hipEventCreate(&e1));
             HIPCHK(c, hipEventRecord(e0, c->stream));
-            hipLaunchKernelGGL(f, dim3(n), dim3(kBlock), lds, c->stream, a, first, n);
-            HIPCHK(c, hipGetLastError());
+            if ((st = launch_coarse(c, mixed, first, n))) return st;
             HIPCHK(c, hipEventRecord(e1, c->stream));
             events->push_back(e0);
             events->push_back(e1);
-        } else {
-            hipLaunchKernelGGL(f, dim3(n), dim3(kBlock), lds, c->stream, a, first, n);
-            HIPCHK(c, hipGetLastError());
-        }
+        } else if ((st = launch_coarse(c, mixed, first, n))) return st;
     }
     hipLaunchKernelGGL(bump_sweep_kernel, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, c->d_sweep, c->R);
     HIPCHK(c, hipGetLastError());
@@ -667,10 +708,8 @@ int sn_mc_replay(sn_ctx *c, int32_t r, int64_t n, const int32_t *site, const dou
     if (mismatches) *mismatches = stats[0];
     if (near_ties) *near_ties = stats[1];
     // the replay's decisions count toward the replica's counters
-    unsigned long long hc[2];
-    HIPCHK(c, hipMemcpy(hc, c->d_counters + 2 * r, sizeof(hc), hipMemcpyDeviceToHost));
-    hc[0] += stats[2]; hc[1] += stats[3];
-    HIPCHK(c, hipMemcpy(c->d_counters + 2 * r, hc, sizeof(hc), hipMemcpyHostToDevice));
+    c->replay_counts[2 * r] += stats[2];
+    c->replay_counts[2 * r + 1] += stats[3];
     void *bufs[] = {ds, dp, du, dd, da, dout, dstat};
     for (void *b : bufs) (void)hipFree(b);
     return SN_OK;
@@ -680,11 +719,19 @@ int sn_counters(sn_ctx *c, int32_t r, uint64_t *accept, uint64_t *reject, uint64
 {
     int st = check_rep(c, r);
     if (st) return st;
-    unsigned long long hc[2];
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(hc, c->d_counters + 2 * r, sizeof(hc), hipMemcpyDeviceToHost));
-    if (accept) *accept = hc[0];
-    if (reject) *reject = hc[1];
+    unsigned long long acc = c->replay_counts[2 * r], rej = c->replay_counts[2 * r + 1];
+    const size_t nt = c->sched.tiles.size();
+    if (nt) {
+        // per-tile slots (each tile's workgroup owns its slot: no atomics)
+        std::vector<unsigned long long> tc(2 * nt);
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(tc.data(), c->d_counters, tc.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < nt; t++)
+            if (c->sched.tiles[t].replica == r) { acc += tc[2 * t]; rej += tc[2 * t + 1]; }
+    }
+    if (accept) *accept = acc;
+    if (reject) *reject = rej;
     if (attempted) *attempted = c->sweeps_done[r] * (unsigned long long)c->reps[r].N;
     return SN_OK;
 }
@@ -692,10 +739,11 @@ int sn_counters(sn_ctx *c, int32_t r, uint64_t *accept, uint64_t *reject, uint64
 int sn_reset_counters(sn_ctx *c)
 {
     if (!c) return SN_ERR_INVALID;
-    std::vector<unsigned long long> z(2 * c->R, 0ull);
+    std::vector<unsigned long long> z(2 * std::max<size_t>(c->sched.tiles.size(), 1), 0ull);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->d_counters, z.data(), z.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
     std::fill(c->sweeps_done.begin(), c->sweeps_done.end(), 0ull);
+    std::fill(c->replay_counts.begin(), c->replay_counts.end(), 0ull);
     return SN_OK;
 }
 

@@ -1,0 +1,271 @@
+// sn_aux_kernels.h -- gather kernels beside the tile sweep (gfx950, FP64):
+// sweep-counter bump, the site_energy batch, sequential replay, energy /
+// adjoint, reductions, moment and the random initial lattice.  Included by
+// sn_api.hip only (non-template kernels: one definition).
+#pragma once
+#include "sn_device.h"
+
+namespace sn {
+__global__ void bump_sweep_kernel(unsigned *sweep_ctr, int R)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) sweep_ctr[r] += 1u;
+}
+
+// Local field pieces of one site by direct gathers from HBM (used by the
+// site_energy batch, the replay and the energy / adjoint kernels).
+struct FieldOut {
+    double hx, hy, hz;  // sum_r L_j (p_j - 3 n (n.p_j)) / d^3
+    double gx, gy, gz;  // sum_{|r|=1} p_j
+};
+
+__device__ __forceinline__ void site_field(const double4 *lat, const RepDev &rep, const PairDev *offs,
+                                           int noff, int x, int y, int z, int first, int stride,
+                                           FieldOut &f)
+{
+    f.hx = f.hy = f.hz = f.gx = f.gy = f.gz = 0.0;
+    for (int i = first; i < noff; i += stride) {
+        const PairDev o = offs[i];
+        int gx = wrap_once(x + o.dx, rep.X), gy = wrap_once(y + o.dy, rep.Y), gz = wrap_once(z + o.dz, rep.Z);
+        double4 t = lat[((long long)gx * rep.Y + gy) * rep.Z + gz];
+        double s = o.nx * t.x + o.ny * t.y + o.nz * t.z;
+        double wl = o.w * t.w;
+        double ts = -3.0 * wl * s;
+        f.hx += wl * t.x + ts * o.nx;
+        f.hy += wl * t.y + ts * o.ny;
+        f.hz += wl * t.z + ts * o.nz;
+        if (o.nn) { f.gx += t.x; f.gy += t.y; f.gz += t.z; }
+    }
+}
+
+__global__ void site_energy_batch_kernel(const double4 *lat, const RepDev *reps, int replica,
+                                         const PairDev *offs, int noff, long long n,
+                                         const int *site, const double *prop, double *dE)
+{
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const RepDev rep = reps[replica];
+    const double4 *L = lat + rep.base;
+    int s = site[k];
+    int z = s % rep.Z, q = s / rep.Z, y = q % rep.Y, x = q / rep.Y;
+    double4 o = L[s];
+    FieldOut f;
+    site_field(L, rep, offs, noff, x, y, z, 0, 1, f);
+    double px = prop[3 * k], py = prop[3 * k + 1], pz = prop[3 * k + 2];
+    double dx = px - o.x, dy = py - o.y, dz = pz - o.z;
+    double e = o.w * (dx * f.hx + dy * f.hy + dz * f.hz) - rep.J * (dx * f.gx + dy * f.gy + dz * f.gz) +
+               (dx * rep.E[0] + dy * rep.E[1] + dz * rep.E[2]);
+    if (rep.K > 0.0) e += -rep.K * (fabs(px) - fabs(o.x) + fabs(py) - fabs(o.y));
+    dE[k] = e;
+}
+
+// Sequential replay of recorded moves: one workgroup, one move at a time;
+// lanes split the neighbour list, a workgroup reduction gives h, g and the
+// rounding scale, lane 0 decides and writes.
+__global__ __launch_bounds__(256) void replay_kernel(double4 *lat, const RepDev *reps, int replica,
+                                                     const PairDev *offs, int noff, long long n,
+                                                     const int *site, const double *prop,
+                                                     const double *u, const unsigned char *u_drawn,
+                                                     const unsigned char *ref_accept,
+                                                     unsigned char *decision,
+                                                     unsigned long long *stats)
+{
+    __shared__ double red[7][256];
+    const int tid = threadIdx.x;
+    const RepDev rep = reps[replica];
+    double4 *L = lat + rep.base;
+    unsigned long long mism = 0, ties = 0, nacc = 0, nrej = 0;
+    for (long long k = 0; k < n; k++) {
+        const int s = site[k];
+        const int z = s % rep.Z, q = s / rep.Z, y = q % rep.Y, x = q / rep.Y;
+        const double4 o = L[s];
+        const double px = prop[3 * k], py = prop[3 * k + 1], pz = prop[3 * k + 2];
+        double v[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int i = tid; i < noff; i += 256) {
+            const PairDev of = offs[i];
+            int gx = wrap_once(x + of.dx, rep.X), gy = wrap_once(y + of.dy, rep.Y), gz = wrap_once(z + of.dz, rep.Z);
+            double4 t = L[((long long)gx * rep.Y + gy) * rep.Z + gz];
+            double s_ = of.nx * t.x + of.ny * t.y + of.nz * t.z;
+            double wl = of.w * t.w;
+            double ts = -3.0 * wl * s_;
+            v[0] += wl * t.x + ts * of.nx;
+            v[1] += wl * t.y + ts * of.ny;
+            v[2] += wl * t.z + ts * of.nz;
+            if (of.nn) { v[3] += t.x; v[4] += t.y; v[5] += t.z; }
+            // rounding scale of the reference's float terms for this neighbour
+            double pn = fabs(px * t.x + py * t.y + pz * t.z), po = fabs(o.x * t.x + o.y * t.y + o.z * t.z);
+            double an = fabs(of.nx * px + of.ny * py + of.nz * pz), ao = fabs(of.nx * o.x + of.ny * o.y + of.nz * o.z);
+            v[6] += fabs(o.w * t.w) * of.w * (pn + po + 3.0 * fabs(s_) * (an + ao));
+            if (of.nn) v[6] += fabs(rep.J) * (pn + po);
+        }
+        for (int c = 0; c < 7; c++) red[c][tid] = v[c];
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (tid < st)
+                for (int c = 0; c < 7; c++) red[c][tid] += red[c][tid + st];
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const double dx = px - o.x, dy = py - o.y, dz = pz - o.z;
+            double e = o.w * (dx * red[0][0] + dy * red[1][0] + dz * red[2][0]) -
+                       rep.J * (dx * red[3][0] + dy * red[4][0] + dz * red[5][0]) +
+                       (dx * rep.E[0] + dy * rep.E[1] + dz * rep.E[2]);
+            double scale = red[6][0] + fabs(px * rep.E[0] + py * rep.E[1] + pz * rep.E[2]) +
+                           fabs(o.x * rep.E[0] + o.y * rep.E[1] + o.z * rep.E[2]);
+            if (rep.K > 0.0) {
+                e += -rep.K * (fabs(px) - fabs(o.x) + fabs(py) - fabs(o.y));
+                scale += rep.K * (fabs(px) + fabs(o.x) + fabs(py) + fabs(o.y));
+            }
+            // float rounding bound of the reference's dE (terms in float,
+            // summed in double, result rounded to float)
+            const double delta = 4e-6 * scale + 2e-7 * fabs(e);
+            const int ref = ref_accept[k] ? 1 : 0;
+            int mine = -1;   // -1: undecidable without a u the reference never drew
+            bool tie = false;
+            if (e < 0.0) mine = 1;
+            else if (u_drawn[k]) mine = exp(-e * rep.beta) > u[k];
+            if (fabs(e) <= delta) tie = true;
+            if (u_drawn[k] && !__builtin_isinf(rep.beta)) {
+                double g = exp(-e * rep.beta);
+                double bound = rep.beta * delta * exp(-(e - delta) * rep.beta);
+                if (fabs(g - u[k]) <= bound) tie = true;
+            }
+            int apply = ref;
+            if (mine != ref) {
+                if (tie) ties++;
+                else mism++;
+            }
+            if (o.w != 0.0) {
+                if (apply) { L[s] = make_double4(px, py, pz, o.w); nacc++; }
+                else nrej++;
+            }
+            decision[k] = (unsigned char)apply;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) { stats[0] = mism; stats[1] = ties; stats[2] = nacc; stats[3] = nrej; }
+}
+
+// Energy and adjoint: one site per lane; per-site gradient dE/dp_i and
+// per-workgroup partial sums of
+//   {E_dd, E_cage, E_field, E_K, dE/dJ, dE/dEx, dE/dEy, dE/dEz, dE/ds, dE/dK}.
+constexpr int kNPart = 10;
+
+__global__ __launch_bounds__(256) void energy_grad_kernel(const double4 *lat, const RepDev *reps,
+                                                          int replica, const PairDev *offs, int noff,
+                                                          double *grad, double *partials)
+{
+    __shared__ double red[kNPart][256];
+    const int tid = threadIdx.x;
+    const RepDev rep = reps[replica];
+    const double4 *L = lat + rep.base;
+    const long long N = (long long)rep.X * rep.Y * rep.Z;
+    long long s = (long long)blockIdx.x * blockDim.x + tid;
+    double v[kNPart];
+    for (int c = 0; c < kNPart; c++) v[c] = 0.0;
+    if (s < N) {
+        int z = (int)(s % rep.Z), q = (int)(s / rep.Z), y = q % rep.Y, x = q / rep.Y;
+        double4 p = L[s];
+        FieldOut f;
+        site_field(L, rep, offs, noff, x, y, z, 0, 1, f);
+        double ph = p.x * f.hx + p.y * f.hy + p.z * f.hz;
+        double pg = p.x * f.gx + p.y * f.gy + p.z * f.gz;
+        double gx = p.w * f.hx - rep.J * f.gx + rep.E[0];
+        double gy = p.w * f.hy - rep.J * f.gy + rep.E[1];
+        double gz = p.w * f.hz - rep.J * f.gz + rep.E[2];
+        double ak = fabs(p.x) + fabs(p.y);
+        if (rep.K > 0.0) {
+            gx -= rep.K * ((p.x > 0.0) - (p.x < 0.0));
+            gy -= rep.K * ((p.y > 0.0) - (p.y < 0.0));
+        }
+        if (grad) { grad[3 * s] = gx; grad[3 * s + 1] = gy; grad[3 * s + 2] = gz; }
+        v[0] = 0.5 * p.w * ph;
+        v[1] = -0.5 * rep.J * pg;
+        v[2] = p.x * rep.E[0] + p.y * rep.E[1] + p.z * rep.E[2];
+        v[3] = rep.K > 0.0 ? -rep.K * ak : 0.0;
+        v[4] = -0.5 * pg;
+        v[5] = p.x; v[6] = p.y; v[7] = p.z;
+        v[8] = p.w * ph;
+        v[9] = rep.K > 0.0 ? -ak : 0.0;
+    }
+    for (int c = 0; c < kNPart; c++) red[c][tid] = v[c];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st)
+            for (int c = 0; c < kNPart; c++) red[c][tid] += red[c][tid + st];
+        __syncthreads();
+    }
+    if (tid < kNPart) partials[(long long)blockIdx.x * kNPart + tid] = red[tid][0];
+}
+
+// Fixed-order sum of per-block partials (deterministic): one workgroup,
+// Neumaier-compensated per lane, then a compensated tree.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double *partials, int nblocks,
+                                                              int width, double *out)
+{
+    __shared__ double s_[256], c_[256];
+    const int tid = threadIdx.x;
+    for (int col = 0; col < width; col++) {
+        double s = 0.0, c = 0.0;
+        for (int b = tid; b < nblocks; b += 256) {
+            double v = partials[(long long)b * width + col];
+            double t = s + v;
+            c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+            s = t;
+        }
+        s_[tid] = s; c_[tid] = c;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (tid < st) {
+                double a = s_[tid], v = s_[tid + st];
+                double t = a + v;
+                c_[tid] += c_[tid + st] + ((fabs(a) >= fabs(v)) ? (a - t) + v : (v - t) + a);
+                s_[tid] = t;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) out[col] = s_[0] + c_[0];
+        __syncthreads();
+    }
+}
+
+// Sum of dipole vectors (partials per block, then reduce_partials_kernel).
+__global__ __launch_bounds__(256) void moment_kernel(const double4 *lat, const RepDev *reps, int replica,
+                                                     double *partials)
+{
+    __shared__ double red[3][256];
+    const int tid = threadIdx.x;
+    const RepDev rep = reps[replica];
+    const long long N = (long long)rep.X * rep.Y * rep.Z;
+    double v0 = 0, v1 = 0, v2 = 0;
+    for (long long s = (long long)blockIdx.x * blockDim.x + tid; s < N; s += (long long)gridDim.x * blockDim.x) {
+        double4 p = lat[rep.base + s];
+        v0 += p.x; v1 += p.y; v2 += p.z;
+    }
+    red[0][tid] = v0; red[1][tid] = v1; red[2][tid] = v2;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) for (int c = 0; c < 3; c++) red[c][tid] += red[c][tid + st];
+        __syncthreads();
+    }
+    if (tid < 3) partials[(long long)blockIdx.x * 3 + tid] = red[tid][0];
+}
+
+// Random initial lattice on the device: unit dipole from the counter RNG
+// (stream tag 0xFFFFFFFF in the sweep slot) and a length drawn from the
+// species table (cumulative prevalence walk, as solid_solution does).
+__global__ void init_random_kernel(double4 *lat, const RepDev *reps, int replica, int nspec,
+                                   const double *len, const double *cum)
+{
+    const RepDev rep = reps[replica];
+    const long long N = (long long)rep.X * rep.Y * rep.Z;
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    double px, py, pz, u;
+    counter_draw(rep.seed, (uint32_t)replica, 0xFFFFFFFFu, (uint32_t)s, 0, px, py, pz, u);
+    int k = 0;
+    while (k + 1 < nspec && u >= cum[k]) k++;
+    lat[rep.base + s] = make_double4(px, py, pz, len[k]);
+}
+
+}  // namespace sn

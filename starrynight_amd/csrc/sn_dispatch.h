@@ -1,0 +1,25 @@
+// sn_dispatch.h -- launch-table lookups for the tile-sweep kernels, whose
+// template instances are compiled in their own translation units
+// (sn_gather_kernels.hip, sn_field_kernels_rc*.hip) so the build runs in parallel.
+#pragma once
+#include "sn_sweep_field.h"
+
+namespace sn {
+
+typedef void (*SweepFn)(SweepArgs, int, int);
+typedef void (*FieldFn)(FieldArgs, int, int);
+
+// 16 lanes share the scatter of one accepted site in the field-update kernel.
+constexpr int kFieldG = 16;
+
+// per-phase gather kernel for G lanes per site, NPL pairs per lane (or null)
+SweepFn pick_sweep(int G, int NPL, bool mixed);
+// field-update kernel for DipoleCutOff rc (1..5) or null
+FieldFn pick_field(int rc, bool two_d, bool mixed);
+FieldFn pick_field_rc1(bool two_d, bool mixed);
+FieldFn pick_field_rc2(bool two_d, bool mixed);
+FieldFn pick_field_rc3(bool two_d, bool mixed);
+FieldFn pick_field_rc4(bool two_d, bool mixed);
+FieldFn pick_field_rc5(bool two_d, bool mixed);
+
+}  // namespace sn

@@ -1,0 +1,16 @@
+// sn_field_kernels_rc3.hip -- field-update tile kernel instances for DipoleCutOff = 3.
+#include <hip/hip_runtime.h>
+
+#include "sn_dispatch.h"
+
+namespace sn {
+
+FieldFn pick_field_rc3(bool two_d, bool mixed)
+{
+    if (two_d) return mixed ? (FieldFn)field_sweep_kernel<3, true, true, kFieldG>
+                            : (FieldFn)field_sweep_kernel<3, true, false, kFieldG>;
+    return mixed ? (FieldFn)field_sweep_kernel<3, false, true, kFieldG>
+                 : (FieldFn)field_sweep_kernel<3, false, false, kFieldG>;
+}
+
+}  // namespace sn

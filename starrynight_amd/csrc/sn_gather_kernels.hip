@@ -1,0 +1,32 @@
+// sn_gather_kernels.hip -- instances of the per-phase gather tile kernel.
+#include <hip/hip_runtime.h>
+
+#include "sn_dispatch.h"
+
+namespace sn {
+
+SweepFn pick_sweep(int G, int NPL, bool mixed)
+{
+#define SN_CASE(g, n)                                                                  \
+    if (G == g && NPL == n) return mixed ? (SweepFn)tile_sweep_kernel<g, n, true>     \
+                                         : (SweepFn)tile_sweep_kernel<g, n, false>;
+    SN_CASE(4, 4) SN_CASE(4, 8) SN_CASE(4, 16) SN_CASE(8, 2) SN_CASE(8, 4) SN_CASE(8, 8) SN_CASE(8, 16)
+    SN_CASE(16, 2) SN_CASE(16, 4) SN_CASE(16, 8) SN_CASE(16, 16) SN_CASE(32, 2) SN_CASE(32, 4)
+    SN_CASE(32, 8)
+#undef SN_CASE
+    return nullptr;
+}
+
+FieldFn pick_field(int rc, bool two_d, bool mixed)
+{
+    switch (rc) {
+    case 1: return pick_field_rc1(two_d, mixed);
+    case 2: return pick_field_rc2(two_d, mixed);
+    case 3: return pick_field_rc3(two_d, mixed);
+    case 4: return pick_field_rc4(two_d, mixed);
+    case 5: return pick_field_rc5(two_d, mixed);
+    }
+    return nullptr;
+}
+
+}  // namespace sn

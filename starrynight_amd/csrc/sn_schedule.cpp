@@ -143,7 +143,7 @@ bool split_axis(int L, int rc, int target, int tmax, std::vector<int> &start, st
 }
 
 bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int target[3],
-                    int colouring_kind, int chunk, int max_region, Schedule &out)
+                    int colouring_kind, int chunk, int max_region, int max_tile, Schedule &out)
 {
     out = Schedule();
     out.rc = rc;
@@ -154,12 +154,14 @@ bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int targ
         if ((g.Z == 1) != out.two_d) { out.error = "replicas mix 2D and 3D lattices"; return false; }
     out.lat = make_sublattice(rc, out.two_d, colouring_kind);
     int hz = out.two_d ? 0 : rc;
-    // largest tile edge whose tile + halo stays within max_region sites
+    // largest tile edge whose tile + halo stays within max_region sites and
+    // whose tile stays within max_tile sites
     int tmax = 1;
     while (true) {
-        int e = tmax + 1 + 2 * rc;
+        int e = tmax + 1 + 2 * rc, t = tmax + 1;
         long long reg = out.two_d ? (long long)e * e : (long long)e * e * e;
-        if (reg > max_region) break;
+        long long vol = out.two_d ? (long long)t * t : (long long)t * t * t;
+        if (reg > max_region || vol > max_tile) break;
         tmax++;
     }
 

@@ -78,9 +78,10 @@ struct Schedule {
 
 // Build the schedule for all replicas.  target[3] = tile edge aimed for per
 // axis; chunk = block size (sites whose proposals are drawn at once);
-// max_region = largest tile + halo (sites) the LDS budget allows.
+// max_region = largest tile + halo (sites) the LDS budget allows;
+// max_tile = largest tile (sites) a workgroup handles.
 bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int target[3],
-                    int colouring_kind, int chunk, int max_region, Schedule &out);
+                    int colouring_kind, int chunk, int max_region, int max_tile, Schedule &out);
 
 // Visit order of replica r for one sweep and its phase boundaries
 // (phase = one (coarse colour, fine colour) pair across all tiles).

@@ -57,6 +57,7 @@ class TuningParams:
     lanes_per_site: int = 0
     fine_colouring: int = 0
     use_graph: int = 0
+    kernel: int = 0   # 0 auto, 1 field-update, 2 per-phase gather
 
     def to_c(self) -> Tuning:
         t = Tuning()
@@ -64,6 +65,7 @@ class TuningParams:
         t.lanes_per_site = self.lanes_per_site
         t.fine_colouring = self.fine_colouring
         t.use_graph = self.use_graph
+        t.kernel = self.kernel
         return t
 
 
@@ -270,3 +272,19 @@ def plan_schedule(replicas: ReplicaParams | Sequence[ReplicaParams], replica: in
     if st != 0:
         raise SnError(st, lib().sn_status_string(st).decode())
     return order, starts
+
+
+def reference_initial_lattice(X: int, Y: int, Z: int, InitialLattice: str = "random", seed: int = None,
+                              T: int = 300, Dipoles=(1.0,), Prevalence=(1.0,)) -> np.ndarray:
+    """The reference's initial state (init_genrand(0xDEADBEEF + T), the
+    InitialLattice pattern, solid_solution), computed on the host; [N, 4]."""
+    if seed is None:
+        seed = (0xDEADBEEF + int(T)) & 0xFFFFFFFF
+    ln = np.ascontiguousarray(Dipoles, np.float64)
+    pv = np.ascontiguousarray(Prevalence, np.float64)
+    out = np.empty((X * Y * Z, 4), np.float64)
+    st = lib().sn_reference_initial_lattice(X, Y, Z, InitialLattice.encode(), seed & 0xFFFFFFFF, len(ln),
+                                            _ptr(ln, C.c_double), _ptr(pv, C.c_double), _ptr(out, C.c_double))
+    if st != 0:
+        raise SnError(st, "invalid initial-lattice request (Prevalence must cover the drawn samples)")
+    return out

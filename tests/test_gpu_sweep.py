@@ -23,7 +23,31 @@ CASES = {
     "16cube_lanes8_minidx": (dict(X=16, Y=16, Z=16, T=300), True, dict(lanes_per_site=8, fine_colouring=2)),
     "16cube_lanes32_cubic": (dict(X=16, Y=16, Z=16, T=300), False, dict(lanes_per_site=32, fine_colouring=1)),
     "18cube_direct_launch": (dict(X=18, Y=18, Z=18, T=300), False, dict(use_graph=-1)),
+    "16cube_gather_kernel": (dict(X=16, Y=16, Z=16, T=300, Efield=(0, 0.05, 0)), True, dict(kernel=2)),
+    "25x25x1_gather_kernel": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=2)),
+    "20cube_rc4_gather_kernel": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300), True, dict(kernel=2)),
+    "odd_tiles_17x19x21": (dict(X=17, Y=19, Z=21, T=350), True, {}),
+    "2d_mixed_30x26": (dict(X=30, Y=26, Z=1, T=200, K=0.2), True, {}),
+    "rc2_12cube": (dict(X=12, Y=12, Z=12, DipoleCutOff=2, T=300), True, {}),
+    "rc5_24cube": (dict(X=24, Y=24, Z=24, DipoleCutOff=5, T=300), False, {}),
 }
+
+
+def test_untileable_lattice_rejects_sweeps_but_gathers(sn, oracle):
+    """16^3 at DipoleCutOff=5 has no tiling with an even tile count per axis
+    and tile >= cut-off: sweeps must raise, site energies must still work."""
+    rp = sn.ReplicaParams(16, 16, 16, DipoleCutOff=5, T=300, seed=3)
+    print_case("untileable", X=16, rc=5)
+    lat = rand_lattice(rp.N, np.random.default_rng(3), mixed=False)
+    with sn.StarryNight([rp]) as ctx:
+        ctx.set_lattice(lat, 0)
+        with pytest.raises(sn.SnError):
+            ctx.MC_sweeps(1)
+        sites = np.arange(0, rp.N, 97, dtype=np.int32)
+        props = rand_lattice(len(sites), np.random.default_rng(4), mixed=False)[:, :3].copy()
+        dE = ctx.site_energy_batch(sites, props, 0)
+    want, _ = oracle_model(oracle, rp).site_energy_batch(lat, sites, props)
+    assert np.allclose(dE, want, rtol=0, atol=1e-10 * max(1.0, np.abs(want).max()))
 
 
 def run_pair(sn, oracle, reps, lats, tuning, nsweeps):
@@ -147,3 +171,18 @@ def test_untileable_lattice_refuses_sweeps_but_gathers_work(sn, oracle):
         dE = ctx.site_energy_batch(np.arange(10, dtype=np.int32), lat[10:20, :3])
     want, _ = oracle_model(oracle, rp).site_energy_batch(lat, np.arange(10), lat[10:20, :3])
     assert np.abs(dE - want).max() < 1e-12 * 100
+
+
+def test_field_and_gather_kernels_agree(sn):
+    """The field-update and per-phase gather kernels visit the same sites with
+    the same draws: after several sweeps the lattices match."""
+    rp = sn.ReplicaParams(24, 24, 24, T=300, Efield=(0.02, 0, 0), seed=31)
+    lat = rand_lattice(rp.N, np.random.default_rng(31), mixed=True)
+    out = []
+    for k in (1, 2):
+        with sn.StarryNight(rp, sn.TuningParams(kernel=k)) as ctx:
+            ctx.set_lattice(lat)
+            ctx.MC_sweeps(6)
+            out.append((ctx.get_lattice(), ctx.counters()))
+    assert np.abs(out[0][0] - out[1][0]).max() <= 1e-12
+    assert out[0][1] == out[1][1]
