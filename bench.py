@@ -35,6 +35,55 @@ def flops_per_update(n_nb: int, n_nn: int) -> int:
     return 19 * n_nb + 3 * n_nn + 40
 
 
+BYTES_PER_UPDATE = 56        # read x,y,z,L (32 B) + write x,y,z (24 B) per attempted update
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(kernel: str, L: int, rc: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (already scaled by the copy-kernel
+    calibration factors stated in that file), or None if there is none."""
+    path = os.path.join(PROFILES, f"{kernel}_{L}cube_rc{rc}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    return vals.get("traffic_bytes_per_launch")
+
+
+def copy_kernel(nbytes: int = 64 << 20, reps: int = 3):
+    """A plain device copy of known size (the FETCH_SIZE / WRITE_SIZE
+    calibration: each launch reads and writes nbytes)."""
+    import torch
+    a = torch.randn(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": reps}), flush=True)
+
+
+def pcie_inclusive(ctx, N, sweeps=10):
+    """The same sweeps with the lattice handed over in host memory: upload,
+    `sweeps` sweeps, download (pageable host buffers, the C ABI's exchange)."""
+    import numpy as np
+    host = ctx.get_lattice()
+    t0 = time.perf_counter()
+    ctx.set_lattice(host)
+    ctx.MC_sweeps(sweeps, sync=False)
+    out = ctx.get_lattice()
+    dt = time.perf_counter() - t0
+    assert out.shape == host.shape and np.isfinite(out).all()
+    return {"site_updates_per_s": sweeps * N / dt, "sweeps": sweeps,
+            "note": "upload + sweeps + download of the 64 MiB FP64 lattice from pageable host memory"}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -81,7 +130,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the headline sweeps (for profiler runs)")
+    ap.add_argument("--copy-kernel", action="store_true",
+                    help="run only a 64 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
+    if args.copy_kernel:
+        copy_kernel()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               "(replicas and slabs would shard across GPUs, see the design notes).", file=sys.stderr)
@@ -116,12 +170,21 @@ def main():
     mean_ms, nl = ctx.time_sweep_kernel(ksweeps)
     updates_per_launch = N * ksweeps / nl
     achieved = updates_per_launch * F / (mean_ms * 1e-3) / 1e12
-    roof = {"bound": "fp64", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-            "kernel": "tile_sweep_kernel", "mean_launch_ms": round(mean_ms, 5),
-            "updates_per_launch": updates_per_launch, "flop_per_update": F}
-    log(f"[bench] tile_sweep_kernel mean {mean_ms * 1e3:.1f} us/launch over {nl} launches -> "
-        f"{achieved:.2f} TFLOP/s ({achieved / PEAK_FP64_TFLOPS:.1%} of FP64 peak)")
+    kname = "field_sweep_kernel" if 1 <= rc <= 5 else "tile_sweep_kernel"
+    traffic = measured_traffic(kname, L, rc)
+    hbm_alg = updates_per_launch * BYTES_PER_UPDATE / (mean_ms * 1e-3) / 1e9
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+            "kernel": kname, "mean_launch_ms": round(mean_ms, 5), "launches_timed": nl,
+            "updates_per_launch": updates_per_launch, "flop_per_update": F,
+            "algorithmic_bytes_per_update": BYTES_PER_UPDATE,
+            "algorithmic_hbm_GBps": round(hbm_alg, 1), "hbm_peak_GBps": PEAK_HBM_GBS,
+            "bound_note": "compute-bound at the FP64 peak (78.6 TFLOP/s, the same for vector and "
+                          "matrix FP64 on MI355X); the kernel runs FP64 VALU, not MFMA (a stencil sum)"}
+    log(f"[bench] {kname} mean {mean_ms * 1e3:.1f} us/launch over {nl} launches -> "
+        f"{achieved:.2f} TFLOP/s ({achieved / PEAK_FP64_TFLOPS:.1%} of FP64 peak), "
+        f"{hbm_alg:.0f} GB/s algorithmic")
+    pcie = None if args.kernel_only else pcie_inclusive(ctx, N)
     ctx.close()
     if args.kernel_only:
         print(json.dumps({"metric": "site-updates/s", "value": rate, "roofline": roof}))
@@ -152,6 +215,7 @@ def main():
         "speedup_vs_cpu_all_cores": (rate / cpu["value"]) if cpu else None,
         "speedup_vs_cpu_one_core": (rate / cpu["single_core"]) if cpu else None,
         "batched": batched,
+        "pcie_inclusive": pcie,
     }
     print(json.dumps(line))
 

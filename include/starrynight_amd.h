@@ -170,6 +170,15 @@ int sn_gradient(sn_ctx *ctx, int32_t replica, double *grad_p, double dparams[6],
  * 506-526). */
 int sn_moment(sn_ctx *ctx, int32_t replica, double M[3]);
 
+/* radial_order_parameter (src/starrynight-analysis.c:528-598) of replica r:
+ * for every r^2 = dx^2+dy^2+dz^2 in 0..80 over offsets in [-9,9]^3, the mean
+ * over all (site, offset) pairs of p_i.p_j (fe) and p_i.p_j - 3(n.p_i)(n.p_j)
+ * (afe), and the pair count.  Bins with count 0 hold 0.  Accumulates in FP64
+ * (the reference accumulates in float); the file the reference appends these
+ * rows to is written by the caller.  Axes shorter than 9 wrap periodically
+ * (the reference indexes out of its array there). */
+int sn_radial_order(sn_ctx *ctx, int32_t replica, double fe[81], double afe[81], int64_t count[81]);
+
 /* ---- schedule introspection (pure host logic, no device work) ----------- */
 
 /* Number of sites N of replica r and number of sequential phases per sweep. */

@@ -541,8 +541,11 @@ void or_radial_order_f64(int X, int Y, int Z, const double *lat, double fe[81], 
                         for (int dz = -C; dz <= C; dz++) {
                             int d2 = dx * dx + dy * dy + dz * dz;
                             if (d2 > C * C || d2 >= 81) continue;
-                            const double *b = lat + 4 * site_index(&m, (x + dx + X) % X,
-                                                                   (y + dy + Y) % Y, (z + dz + Z) % Z);
+                            /* + 9L instead of the reference's + L: the same site
+                             * where the reference's index is valid (L >= 9, or
+                             * Z == 1), a periodic wrap where it is not */
+                            const double *b = lat + 4 * site_index(&m, (x + dx + 9 * X) % X,
+                                                                   (y + dy + 9 * Y) % Y, (z + dz + 9 * Z) % Z);
                             double f = dotd(a, b);
                             double d = sqrt((double)d2);
                             if (d == 0) d = 1;

@@ -1,0 +1,91 @@
+"""torch.autograd face of the adjoint kernel (sn_gradient).
+
+    E = lattice_energy(p, lengths, J, Efield, s, model)
+
+p (N,3) are the dipole orientations, lengths (N,) the per-site lengths (0 =
+vacancy, not differentiated), J the cage strain, Efield (3,) the field and s a
+dipole-strength factor multiplying every length.  The energy is the
+Hamiltonian whose single-site differences are the reference's site_energy
+(src/starrynight-montecarlo-core.c:76-141):
+
+    E = 1/2 sum_i sum_r sL_i sL_j [p_i.p_j - 3(n.p_i)(n.p_j)]/d^3
+        - J/2 sum_<ij> p_i.p_j + sum_i p_i.Efield - [K>0] K sum_i (|p_ix| + |p_iy|)
+
+Forward and backward both run on the GPU in one pass of the adjoint kernel
+(energy, dE/dp_i on every site, dE/d(J, Efield, s)); backward only scales the
+stored gradients by the incoming grad.  K and the geometry are fixed by the
+model's ReplicaParams.  There is no CPU path.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .lattice import ReplicaParams, StarryNight
+
+
+class EnergyModel:
+    """A one-replica context the energy function evaluates on (geometry,
+    DipoleCutOff and K come from `rp`; J and Efield are inputs)."""
+
+    def __init__(self, rp: ReplicaParams):
+        self.rp = rp
+        self.ctx = StarryNight(rp)
+
+    def close(self):
+        self.ctx.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def evaluate(self, p: np.ndarray, lengths: np.ndarray, J: float, E: np.ndarray, s: float):
+        """Energy, dE/dp (N,3) and dE/d(J, Ex, Ey, Ez, s) at the given inputs."""
+        N = self.rp.N
+        lat = np.empty((N, 4), np.float64)
+        lat[:, :3] = p
+        lat[:, 3] = lengths * s
+        self.ctx.set_params(0, self.rp.T, tuple(float(v) for v in E), float(J))
+        self.ctx.set_lattice(lat)
+        g, dp, e = self.ctx.gradient()
+        # dE/ds: the kernel returns d/dt E(t * lengths * s) at t = 1 = s * dE/ds
+        if s != 0.0:
+            ds = dp[4] / s
+        else:   # E_dd is quadratic in s: dE/ds = 2 s E_dd(lengths) = 0 at s = 0
+            ds = 0.0
+        return e, g, np.array([dp[0], dp[1], dp[2], dp[3], ds])
+
+
+class _LatticeEnergy(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, p, lengths, J, E, s, model):
+        pn = p.detach().to("cpu", torch.float64).numpy()
+        ln = lengths.detach().to("cpu", torch.float64).numpy()
+        En = E.detach().to("cpu", torch.float64).numpy()
+        e, g, dparams = model.evaluate(pn, ln, float(J), En, float(s))
+        ctx.save_for_backward(torch.from_numpy(g).to(p.device),
+                              torch.from_numpy(dparams).to(p.device))
+        return torch.tensor(e, dtype=torch.float64, device=p.device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        g, dparams = ctx.saved_tensors
+        gp = gout * g
+        gJ = (gout * dparams[0]).reshape(())
+        gE = gout * dparams[1:4]
+        gs = (gout * dparams[4]).reshape(())
+        return gp, None, gJ, gE, gs, None
+
+
+def lattice_energy(p: torch.Tensor, lengths: torch.Tensor, J: torch.Tensor, E: torch.Tensor,
+                   s: torch.Tensor, model: EnergyModel) -> torch.Tensor:
+    """Total lattice energy (FP64) with gradients to p, J, E and s."""
+    if p.dtype != torch.float64:
+        raise TypeError("p must be float64 (the path runs in FP64 throughout)")
+    J, E, s = (torch.as_tensor(v, dtype=torch.float64, device=p.device) for v in (J, E, s))
+    N = model.rp.N
+    if p.shape != (N, 3) or lengths.shape != (N,) or E.shape != (3,):
+        raise ValueError(f"expected p ({N},3), lengths ({N},), E (3,)")
+    return _LatticeEnergy.apply(p, lengths, J, E, s, model)

@@ -47,6 +47,9 @@ struct sn_ctx {
     int *d_class_start = nullptr, *d_class_ri = nullptr, *d_chunk_col = nullptr;
     int *d_class_tile = nullptr, *d_tile_coord = nullptr;
     PairDev *d_pairs = nullptr, *d_offs = nullptr;
+    RadialOff *d_radial = nullptr;   // radial-order offsets sorted by r^2 (built on first use)
+    int *d_radial_bins = nullptr;
+    std::vector<int> radial_bins;
     unsigned *d_sweep = nullptr;
     unsigned long long *d_counters = nullptr;   // 2 per tile (accepted, rejected)
     std::vector<unsigned long long> replay_counts;  // 2 per replica, from sn_mc_replay
@@ -358,7 +361,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if ((st = upload(c, &c->d_sweep, zeros_u))) return st;
     std::vector<unsigned long long> zeros_c(2 * std::max<size_t>(c->sched.tiles.size(), 1), 0ull);
     if ((st = upload(c, &c->d_counters, zeros_c))) return st;
-    HIPCHK(c, hipMalloc((void **)&c->d_result, 64 * sizeof(double)));
+    HIPCHK(c, hipMalloc((void **)&c->d_result, 256 * sizeof(double)));
     c->sweeps_done.assign(c->R, 0);
     c->replay_counts.assign(2 * c->R, 0ull);
 
@@ -389,7 +392,7 @@ int sn_destroy(sn_ctx *c)
     free_graph(c);
     void *bufs[] = {c->d_lat, c->d_reps, c->d_tiles, c->d_shapes, c->d_class_start, c->d_class_ri,
                     c->d_class_tile, c->d_tile_coord, c->d_chunk_col, c->d_pairs, c->d_offs, c->d_sweep,
-                    c->d_counters, c->d_partials, c->d_result};
+                    c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -827,6 +830,61 @@ int sn_moment(sn_ctx *c, int32_t r, double M[3])
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(M, c->d_result, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return SN_OK;
+}
+
+// Offsets of radial_order_parameter (src/starrynight-analysis.c:552-561): every
+// (dx,dy,dz) in [-9,9]^3 with r^2 <= 80, grouped by r^2; n = r/|r| in FP64.
+// The reference's r^2 = 81 bin is accumulated but never zeroed or printed
+// (its arrays hold 82 entries, its loops run to 80), so it is left out.
+static int ensure_radial(sn_ctx *c)
+{
+    if (c->d_radial) return SN_OK;
+    std::vector<RadialOff> offs;
+    std::vector<int> start(kRadialBins + 1, 0);
+    for (int b = 0; b < kRadialBins; b++) {
+        start[b] = (int)offs.size();
+        for (int dx = -9; dx <= 9; dx++)
+            for (int dy = -9; dy <= 9; dy++)
+                for (int dz = -9; dz <= 9; dz++) {
+                    if (dx * dx + dy * dy + dz * dz != b) continue;
+                    double d = b ? std::sqrt((double)b) : 1.0;
+                    offs.push_back(RadialOff{dx, dy, dz, 0, dx / d, dy / d, dz / d, 0.0});
+                }
+    }
+    start[kRadialBins] = (int)offs.size();
+    HIPCHK(c, hipMalloc((void **)&c->d_radial, offs.size() * sizeof(RadialOff)));
+    HIPCHK(c, hipMalloc((void **)&c->d_radial_bins, start.size() * sizeof(int)));
+    HIPCHK(c, hipMemcpy(c->d_radial, offs.data(), offs.size() * sizeof(RadialOff), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_radial_bins, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->radial_bins = start;
+    return SN_OK;
+}
+
+int sn_radial_order(sn_ctx *c, int32_t r, double fe[81], double afe[81], int64_t count[81])
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!fe || !afe || !count) return fail(c, SN_ERR_INVALID, "null output");
+    if ((st = ensure_radial(c))) return st;
+    const RepHost &h = c->reps[r];
+    int nblk = (int)std::min<long long>((h.N + 255) / 256, 2048);
+    if ((st = ensure_partials(c, (size_t)nblk * 2 * kRadialBins))) return st;
+    hipLaunchKernelGGL(radial_order_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
+                       c->d_radial, c->d_radial_bins, c->d_partials);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk,
+                       2 * kRadialBins, c->d_result);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double sums[2 * kRadialBins];
+    HIPCHK(c, hipMemcpy(sums, c->d_result, sizeof(sums), hipMemcpyDeviceToHost));
+    for (int b = 0; b < kRadialBins; b++) {
+        const long long cnt = h.N * (long long)(c->radial_bins[b + 1] - c->radial_bins[b]);
+        count[b] = cnt;
+        fe[b] = cnt ? sums[2 * b] / (double)cnt : 0.0;
+        afe[b] = cnt ? sums[2 * b + 1] / (double)cnt : 0.0;
+    }
     return SN_OK;
 }
 

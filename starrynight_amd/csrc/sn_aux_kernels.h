@@ -268,4 +268,67 @@ __global__ void init_random_kernel(double4 *lat, const RepDev *reps, int replica
     lat[rep.base + s] = make_double4(px, py, pz, len[k]);
 }
 
+// radial_order_parameter (src/starrynight-analysis.c:528-598) in FP64.  Over
+// every site i and every offset r with |r|^2 <= 80 (the dz range runs even
+// when Z == 1, where it wraps onto the same plane, as in the reference):
+//   FE[r^2]  += p_i.p_j,   AFE[r^2] += p_i.p_j - 3 (n.p_i)(n.p_j),  j = i + r
+// with n = r/|r| (n = 0 at r = 0).  Offsets arrive sorted by r^2 with
+// bin_start[82]; the loop over them is uniform across the wave, so the
+// offset table is read through scalar loads.  One lane per site; each wave
+// reduces a bin and its lane 0 adds to that wave's own LDS row, so no LDS
+// atomics; a block writes 162 partial sums.
+struct RadialOff {
+    int dx, dy, dz, pad;
+    double nx, ny, nz, pad2;
+};
+constexpr int kRadialBins = 81;
+
+__global__ __launch_bounds__(256) void radial_order_kernel(const double4 *lat, const RepDev *reps, int replica,
+                                                           const RadialOff *offs, const int *bin_start,
+                                                           double *partials)
+{
+    __shared__ double hist[4][2 * kRadialBins];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < 4 * 2 * kRadialBins; i += 256) (&hist[0][0])[i] = 0.0;
+    __syncthreads();
+    const RepDev rep = reps[replica];
+    const int X = rep.X, Y = rep.Y, Z = rep.Z;
+    const long long N = (long long)X * Y * Z;
+    // x + dx + 9X >= 0 for |dx| <= 9: one unsigned modulo wraps any axis length
+    const unsigned bx = 9u * X, by = 9u * Y, bz = 9u * Z;
+    for (long long s0 = (long long)blockIdx.x * 256; s0 < N; s0 += (long long)gridDim.x * 256) {
+        const long long s = s0 + tid;
+        const bool act = s < N;
+        const long long sc = act ? s : 0;
+        const int z = (int)(sc % Z), y = (int)((sc / Z) % Y), x = (int)(sc / ((long long)Y * Z));
+        double4 a = lat[rep.base + sc];
+        if (!act) a = make_double4(0.0, 0.0, 0.0, 0.0);   // contributes exact zeros
+        for (int b = 0; b < kRadialBins; b++) {
+            double fe = 0.0, afe = 0.0;
+            const int k1 = bin_start[b + 1];
+            for (int k = bin_start[b]; k < k1; k++) {
+                const RadialOff o = offs[k];
+                const unsigned xx = (unsigned)(x + o.dx + (int)bx) % (unsigned)X;
+                const unsigned yy = (unsigned)(y + o.dy + (int)by) % (unsigned)Y;
+                const unsigned zz = (unsigned)(z + o.dz + (int)bz) % (unsigned)Z;
+                const double4 q = lat[rep.base + ((long long)xx * Y + yy) * Z + zz];
+                const double f = a.x * q.x + a.y * q.y + a.z * q.z;
+                const double na = o.nx * a.x + o.ny * a.y + o.nz * a.z;
+                const double nq = o.nx * q.x + o.ny * q.y + o.nz * q.z;
+                fe += f;
+                afe += f - 3.0 * na * nq;
+            }
+            fe = group_sum<64>(fe);
+            afe = group_sum<64>(afe);
+            if (lane == 0) {
+                hist[wave][2 * b] += fe;
+                hist[wave][2 * b + 1] += afe;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < 2 * kRadialBins; i += 256)
+        partials[(long long)blockIdx.x * 2 * kRadialBins + i] = (hist[0][i] + hist[1][i]) + (hist[2][i] + hist[3][i]);
+}
+
 }  // namespace sn

@@ -211,6 +211,15 @@ class StarryNight:
         self._chk(lib().sn_moment(self._h, replica, _ptr(M, C.c_double)))
         return M
 
+    def radial_order(self, replica: int = 0):
+        """radial_order_parameter (src/starrynight-analysis.c:528-598):
+        (fe[81], afe[81], count[81]) indexed by r^2."""
+        fe, afe = np.empty(81, np.float64), np.empty(81, np.float64)
+        cnt = np.empty(81, np.int64)
+        self._chk(lib().sn_radial_order(self._h, replica, _ptr(fe, C.c_double), _ptr(afe, C.c_double),
+                                        _ptr(cnt, C.c_int64)))
+        return fe, afe, cnt
+
     # -- schedule / timing
     def schedule(self, replica: int = 0):
         n, ph = C.c_int64(0), C.c_int32(0)

@@ -74,3 +74,20 @@ def test_moment_and_counters(sn):
         ctx.set_lattice(lat)
         M = ctx.moment()
     assert np.abs(M - lat[:, :3].sum(0)).max() < 1e-11
+
+
+@pytest.mark.parametrize("dims,mixed", [((20, 20, 20), True), ((25, 25, 1), False), ((12, 10, 9), True),
+                                        ((7, 7, 7), False), ((48, 48, 48), False)])
+def test_radial_order_parity(sn, oracle, dims, mixed):
+    """sn_radial_order vs the oracle's radial_order_parameter restatement:
+    counts exact, means to 1e-12 (only the summation order differs)."""
+    rp = sn.ReplicaParams(*dims)
+    print_case("radial_order", dims=dims, mixed=mixed)
+    lat = rand_lattice(rp.N, np.random.default_rng(sum(dims)), mixed=mixed)
+    with sn.StarryNight(rp) as ctx:
+        ctx.set_lattice(lat)
+        fe, afe, cnt = ctx.radial_order()
+    o = oracle_model(oracle, rp)
+    few, afew, cntw = o.radial_order(lat)
+    assert np.array_equal(cnt, cntw)
+    assert np.abs(fe - few).max() <= 1e-12 and np.abs(afe - afew).max() <= 1e-12

@@ -328,3 +328,57 @@ def test_radial_order_count_invariant(oracle):
             assert count == cnt[r2]
     # r = 0 bin: FE correlation is |p|^2 = 1
     assert abs(fe[0] - 1.0) < 1e-12 and cnt[0] == N
+
+
+def _shell(b):
+    return [(x, y, z) for x in range(-9, 10) for y in range(-9, 10) for z in range(-9, 10)
+            if x * x + y * y + z * z == b]
+
+
+@pytest.mark.parametrize("dims", [(10, 10, 10), (12, 11, 1)])
+def test_radial_order_ferroelectric_closed_form(oracle, dims):
+    """All dipoles along x: FE = 1 in every bin; AFE = 1 - 3<n_x^2> = 0 for
+    r^2 > 0 (each shell is symmetric under coordinate permutations, so
+    <n_x^2> = 1/3) and AFE = 1 at r = 0 (n = 0)."""
+    X, Y, Z = dims
+    lat = np.zeros((X * Y * Z, 4)); lat[:, 0] = 1.0; lat[:, 3] = 1.0
+    fe, afe, cnt = oracle.Oracle(X, Y, Z, 3).radial_order(lat)
+    occ = cnt > 0
+    assert np.abs(fe[occ] - 1.0).max() < 1e-12
+    assert abs(afe[0] - 1.0) < 1e-12 and np.abs(afe[1:][occ[1:]]).max() < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(10, 10, 10), (10, 9, 1), (8, 9, 9)])
+def test_radial_order_layered_closed_form(oracle, dims):
+    """p = (-1)^x x-hat (X even): a pair at offset r correlates as (-1)^dx,
+    so FE[r^2] = <(-1)^dx> and AFE[r^2] = <(-1)^dx (1 - 3 dx^2/r^2)> over the
+    shell; with Z == 1 the dz range still runs (onto the same plane), and an
+    axis shorter than 9 wraps periodically.  Also checks the pair counts."""
+    X, Y, Z = dims
+    N = X * Y * Z
+    x = np.arange(N) // (Y * Z)
+    lat = np.zeros((N, 4)); lat[:, 0] = np.where(x % 2 == 0, 1.0, -1.0); lat[:, 3] = 1.0
+    fe, afe, cnt = oracle.Oracle(X, Y, Z, 3).radial_order(lat)
+    for b in range(81):
+        sh = _shell(b)
+        assert cnt[b] == N * len(sh)
+        if not sh:
+            continue
+        sgn = np.array([(-1) ** (dx % 2) for dx, _, _ in sh], float)
+        nx2 = np.array([dx * dx / b if b else 0.0 for dx, _, _ in sh])
+        assert abs(fe[b] - sgn.mean()) < 1e-12
+        assert abs(afe[b] - (sgn * (1 - 3 * nx2)).mean()) < 1e-12
+
+
+def test_library_exports_every_declared_symbol(sn):
+    """libstarrynight_amd.so loads (no device work) and exports every function
+    include/starrynight_amd.h declares; the Python binding lists the same set."""
+    import ctypes
+    import re
+    hdr = open(os.path.join(os.path.dirname(GOLDEN), "..", "include", "starrynight_amd.h")).read()
+    declared = set(re.findall(r"^(?:int|const char \*)\s*(sn_\w+)\s*\(", hdr, re.M))
+    assert len(declared) >= 25
+    lib = ctypes.CDLL(sn.LIB_PATH)
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared == set(sn.EXPORTS)
