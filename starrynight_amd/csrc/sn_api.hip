@@ -45,7 +45,9 @@ struct sn_ctx {
     TileDev *d_tiles = nullptr;
     ShapeDev *d_shapes = nullptr;
     int *d_class_start = nullptr, *d_class_ri = nullptr, *d_chunk_col = nullptr;
-    int *d_class_tile = nullptr, *d_tile_coord = nullptr;
+    int *d_field_steps = nullptr, *d_tile_coord = nullptr;
+    int max_steps = 0;   // longest step table of a tile shape (field kernel)
+    int max_tile_padded = 0;   // largest padded tile (field kernel's tile-order arrays)
     PairDev *d_pairs = nullptr, *d_offs = nullptr;
     RadialOff *d_radial = nullptr;   // radial-order offsets sorted by r^2 (built on first use)
     int *d_radial_bins = nullptr;
@@ -178,14 +180,14 @@ static int field_region_doubles(const sn_ctx *c) { return c->field_region + 2; }
 static int field_phaseA(const sn_ctx *c, bool mixed)
 {
     int region = (mixed ? 4 : 3) * field_region_doubles(c);
-    int compact = (mixed ? 11 : 10) * c->max_tile_sites;
+    int compact = (mixed ? 11 : 10) * c->max_tile_padded;
     return std::max(region, compact);
 }
 
 static size_t field_lds(const sn_ctx *c, bool mixed)
 {
     return (size_t)field_phaseA(c, mixed) * sizeof(double) +
-           (size_t)(2 * c->max_tile_sites + c->max_cls + 1) * sizeof(int);
+           (size_t)(c->max_tile_padded + c->max_steps * (kBlock / kFieldG + 1)) * sizeof(int);
 }
 
 extern "C" {
@@ -317,7 +319,8 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     for (const Tile &t : c->sched.tiles) td.push_back({t.replica, t.x0, t.y0, t.z0, t.shape, 0, 0, 0});
     if ((st = upload(c, &c->d_tiles, td))) return st;
     std::vector<ShapeDev> sd;
-    std::vector<int> cstart, cri, chunks, ctile, tcoord;
+    std::vector<int> cstart, cri, chunks, fsteps, tcoord;
+    constexpr int NG = kBlock / kFieldG;   // lane groups of the field kernel
     const int hz = c->two_d ? 0 : c->rc;
     for (const TileShape &s : c->sched.shapes) {
         ShapeDev d{};
@@ -330,17 +333,40 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         // by fine colour: region index (tile + halo) and tile-local index of
         // each site; by tile index: packed local coordinates
         const int RY = s.ty + 2 * c->rc, RZ = s.tz + 2 * hz;
-        const int ntile = s.tx * s.ty * s.tz;
         for (size_t k = 0; k < s.class_sites.size(); k++) {
             int li = s.class_sites[k];
             int lz = li % s.tz, ly = (li / s.tz) % s.ty, lx = li / (s.tz * s.ty);
             cri.push_back(((lx + c->rc) * RY + (ly + c->rc)) * RZ + (lz + hz));
-            ctile.push_back(li);
         }
-        for (int li = 0; li < ntile; li++) {
-            int lz = li % s.tz, ly = (li / s.tz) % s.ty, lx = li / (s.tz * s.ty);
-            tcoord.push_back(lx | (ly << 10) | (lz << 20));
+        // field kernel: tile-order arrays use padded strides (z in 3D, y in
+        // 2D) so that a scatter's targets spread over the LDS banks
+        auto pad = [](int t) { return (t % 2 == 0) ? t + 3 : t + 2; };
+        d.tzp = c->two_d ? 1 : pad(s.tz);
+        d.typ = c->two_d ? pad(s.ty) : s.ty;
+        d.coord_off = (int)tcoord.size();
+        const int npad = s.tx * d.typ * d.tzp;
+        for (int i = 0; i < npad; i++) {
+            const int lz = i % d.tzp, ly = (i / d.tzp) % d.typ, lx = i / (d.tzp * d.typ);
+            tcoord.push_back(lz < s.tz && ly < s.ty ? (lx | (ly << 10) | (lz << 20)) : -1);
         }
+        c->max_tile_padded = std::max(c->max_tile_padded, npad);
+        auto padded = [&](int li) {
+            const int lz = li % s.tz, ly = (li / s.tz) % s.ty, lx = li / (s.tz * s.ty);
+            return (lx * d.typ + ly) * d.tzp + lz;
+        };
+        // field kernel steps: each colour in blocks of NG sites; per step a
+        // barrier flag (last block of the colour) and NG padded tile indices (-1: none)
+        d.step_off = (int)fsteps.size();
+        d.nsteps = 0;
+        for (int col = 0; col < s.ncol; col++) {
+            const int a0 = s.class_start[col], a1 = s.class_start[col + 1];
+            for (int b0 = a0; b0 < a1; b0 += NG) {
+                fsteps.push_back(b0 + NG >= a1 ? 1 : 0);
+                for (int g = 0; g < NG; g++) fsteps.push_back(b0 + g < a1 ? padded(s.class_sites[b0 + g]) : -1);
+                d.nsteps++;
+            }
+        }
+        c->max_steps = std::max(c->max_steps, d.nsteps);
         chunks.insert(chunks.end(), s.chunk_start.begin(), s.chunk_start.end());
         c->max_tile_sites = std::max(c->max_tile_sites, s.tx * s.ty * s.tz);
         const int RZp = c->two_d ? RZ : (RZ | 1);
@@ -352,7 +378,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if ((st = upload(c, &c->d_shapes, sd))) return st;
     if ((st = upload(c, &c->d_class_start, cstart))) return st;
     if ((st = upload(c, &c->d_class_ri, cri))) return st;
-    if ((st = upload(c, &c->d_class_tile, ctile))) return st;
+    if ((st = upload(c, &c->d_field_steps, fsteps))) return st;
     if ((st = upload(c, &c->d_tile_coord, tcoord))) return st;
     if ((st = upload(c, &c->d_chunk_col, chunks))) return st;
     if ((st = upload(c, &c->d_pairs, hp))) return st;
@@ -391,7 +417,7 @@ int sn_destroy(sn_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph(c);
     void *bufs[] = {c->d_lat, c->d_reps, c->d_tiles, c->d_shapes, c->d_class_start, c->d_class_ri,
-                    c->d_class_tile, c->d_tile_coord, c->d_chunk_col, c->d_pairs, c->d_offs, c->d_sweep,
+                    c->d_field_steps, c->d_tile_coord, c->d_chunk_col, c->d_pairs, c->d_offs, c->d_sweep,
                     c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -566,11 +592,12 @@ static int launch_coarse(sn_ctx *c, bool mixed, int first, int n)
     if (c->use_field) {
         FieldArgs fa;
         fa.s = sweep_args(c);
-        fa.class_tile = c->d_class_tile;
+        fa.steps = c->d_field_steps;
         fa.tile_coord = c->d_tile_coord;
         fa.tile_counts = c->d_counters;
         fa.phaseA_doubles = field_phaseA(c, mixed);
         fa.region_doubles = field_region_doubles(c);
+        fa.tile_doubles = c->max_tile_padded;
         hipLaunchKernelGGL(pick_field(c->rc, c->two_d, mixed), dim3(n), dim3(kBlock), field_lds(c, mixed),
                            c->stream, fa, first, n);
     } else {

@@ -23,6 +23,9 @@ struct TileDev {
 struct ShapeDev {
     int tx, ty, tz, ncol;
     int nchunks, class_off, chunk_off, sites_off;
+    int nsteps, step_off;     // field kernel's step table (see sn_sweep_field.h)
+    int typ, tzp, coord_off;  // field kernel's padded tile-order strides and coordinate table
+    int pad_;
 };
 
 // One +-r neighbour pair (sweep kernel) or one offset (gather kernels).

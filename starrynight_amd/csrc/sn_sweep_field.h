@@ -256,8 +256,9 @@ constexpr int n_pairs()
 
 struct FieldArgs {
     SweepArgs s;
-    const int *class_tile;    // per shape (sites_off): tile-local index of each site, by fine colour
-    const int *tile_coord;    // per shape (sites_off): packed local coords lx | ly << 10 | lz << 20
+    const int *steps;         // per shape (step_off): per step, barrier flag + NGROUPS tile indices
+    const int *tile_coord;    // per shape (coord_off), by padded tile index: lx | ly << 10 | lz << 20, -1 in the padding
+    int tile_doubles;         // stride of the tile-order arrays (largest padded tile)
     unsigned long long *tile_counts;  // 2 per tile (accepted, rejected), accumulated
     int phaseA_doubles;       // LDS doubles reserved for the staged region / compact arrays
     int region_doubles;       // stride between the staged x, y, z (, length) arrays
@@ -269,7 +270,6 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
     constexpr int HZ = TWO_D ? 0 : RC;
     constexpr int S = TWO_D ? 1 : 2;                 // sites per gather item (along z)
     constexpr int NP = n_pairs<RC, TWO_D>();
-    constexpr int NPL = (NP + G - 1) / G;            // scatter pairs per lane
     constexpr int NGROUPS = kBlock / G;
     const SweepArgs &a = fa.s;
     extern __shared__ double smem[];
@@ -286,8 +286,11 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
     // below, a wave's column loads then spread evenly over the LDS banks
     const int RZp = TWO_D ? RZ : (RZ | 1);
     const int sY = RZp, sX = RY * RZp;
-    const int ntile = sh.tx * sh.ty * sh.tz;
-    const int MT = a.max_tile_sites;
+    // tile-order arrays use the padded index (lx * TYP + ly) * TZP + lz: the
+    // padded strides spread a scatter's targets over the LDS banks
+    const int TYP = sh.typ, TZP = sh.tzp;
+    const int npad = sh.tx * TYP * TZP;
+    const int MT = fa.tile_doubles;
 
     // ---- LDS: [phase A / phase B area][tables].  Region arrays are padded by
     // two doubles: the phantom second site of the last item of an odd-length
@@ -303,14 +306,10 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
     double *cqx = cpz + MT, *cqy = cqx + MT, *cqz = cqy + MT, *cthr = cqz + MT;
     double *cl = cthr + MT;                             // MIXED only
     int *s_coord = (int *)(smem + fa.phaseA_doubles);   // tile order
-    int *s_ctile = s_coord + MT;                          // class order -> tile index
-    int *s_cls = s_ctile + MT;
+    int *s_step = s_coord + MT;                           // step table, (NGROUPS + 1) ints per step
 
-    for (int i = tid; i < ntile; i += kBlock) {
-        s_coord[i] = fa.tile_coord[sh.sites_off + i];
-        s_ctile[i] = fa.class_tile[sh.sites_off + i];
-    }
-    for (int i = tid; i <= sh.ncol; i += kBlock) s_cls[i] = a.class_start[sh.class_off + i];
+    for (int i = tid; i < npad; i += kBlock) s_coord[i] = fa.tile_coord[sh.coord_off + i];
+    for (int i = tid; i < sh.nsteps * (NGROUPS + 1); i += kBlock) s_step[i] = fa.steps[sh.step_off + i];
 
     // ---- stage tile + halo ----
     const double4 *lat = a.lat + rep.base;
@@ -371,14 +370,14 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
         o.f0[1] = cA * A.hy - J * A.gy + rep.E[1];
         o.f0[2] = cA * A.hz - J * A.gz + rep.E[2];
         o.p0[0] = pA[0]; o.p0[1] = pA[1]; o.p0[2] = pA[2]; o.p0[3] = pA[3];
-        o.c0 = (lx * sh.ty + ly) * sh.tz + lz;
+        o.c0 = (lx * TYP + ly) * TZP + lz;
         if (S == 2 && lz + 1 < sh.tz) {
             const double cB = MIXED ? pB[3] : c_nm;
             o.f1[0] = cB * B.hx - J * B.gx + rep.E[0];
             o.f1[1] = cB * B.hy - J * B.gy + rep.E[1];
             o.f1[2] = cB * B.hz - J * B.gz + rep.E[2];
             o.p1[0] = pB[0]; o.p1[1] = pB[1]; o.p1[2] = pB[2]; o.p1[3] = pB[3];
-            o.c1 = (lx * sh.ty + ly) * sh.tz + lz + 1;
+            o.c1 = (lx * TYP + ly) * TZP + lz + 1;
         }
     };
     ItemOut o0, o1;
@@ -403,8 +402,9 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
     // ---- proposals of every tile site (tile order) ----
     const unsigned sweep = a.sweep_ctr[tile.replica];
     const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
-    for (int c = tid; c < ntile; c += kBlock) {
+    for (int c = tid; c < npad; c += kBlock) {
         const int pc = s_coord[c];
+        if (pc < 0) continue;
         const int lx = pc & 1023, ly = (pc >> 10) & 1023, lz = pc >> 20;
         unsigned gsite = (unsigned)(tile_base + ((long long)lx * rep.Y + ly) * rep.Z + lz);
         double px, py, pz, u;
@@ -413,81 +413,175 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
         cthr[c] = metropolis_threshold(u, rep.beta);
     }
 
-    // ---- this lane's scatter pairs ----
+    // ---- this lane's scatter pairs: pair j = lane + 64 q of the +-r list ----
+    // An accepted site i adds to the field of every tile site t = i +- r
+    //   c_t T_r (L_i dp) - J dp [|r| = 1],  T_r m = w m - 3 w (n.m) n,
+    // written per pair as A dp + (B (n.dp)) n: A = c w - J [nn], B = -3 c w
+    // with c = L0^2 for uniform lengths; with mixed lengths A = w, B = -3 w and
+    // the result is scaled by c_t L_i and -J dp [nn] added per target.
+    constexpr int PPL = (NP + 63) / 64;
+    const int lane = tid & 63;
     const int part = tid % G, group = tid / G;
-    int pdx[NPL], pdy[NPL], pdz[NPL];
-    double pnx[NPL], pny[NPL], pnz[NPL], pw[NPL], pnn[NPL];
+    int pdx[PPL], pdy[PPL], pdz[PPL], poff[PPL];
+    double pnx[PPL], pny[PPL], pnz[PPL], pA[PPL], pB[PPL], pjn[PPL];
 #pragma unroll
-    for (int m = 0; m < NPL; m++) {
-        const int j = part + m * G;
+    for (int q = 0; q < PPL; q++) {
+        const int j = lane + 64 * q;
         if (j < NP) {
             PairDev p = a.pairs[j];
-            pdx[m] = p.dx; pdy[m] = p.dy; pdz[m] = p.dz;
-            pnx[m] = p.nx; pny[m] = p.ny; pnz[m] = p.nz; pw[m] = p.w; pnn[m] = p.nn ? 1.0 : 0.0;
+            pdx[q] = p.dx; pdy[q] = p.dy; pdz[q] = p.dz;
+            pnx[q] = p.nx; pny[q] = p.ny; pnz[q] = p.nz;
+            pjn[q] = p.nn ? -J : 0.0;
+            if constexpr (MIXED) { pA[q] = p.w; pB[q] = -3.0 * p.w; }
+            else { pA[q] = c_nm * p.w + pjn[q]; pB[q] = -3.0 * c_nm * p.w; }
         } else {
-            pdx[m] = 1 << 20; pdy[m] = 0; pdz[m] = 0;   // never inside a tile
-            pnx[m] = pny[m] = pnz[m] = pw[m] = pnn[m] = 0.0;
+            pdx[q] = 1 << 20; pdy[q] = 0; pdz[q] = 0;   // never inside a tile
+            pnx[q] = pny[q] = pnz[q] = pA[q] = pB[q] = pjn[q] = 0.0;
         }
+        poff[q] = (pdx[q] * TYP + pdy[q]) * TZP + pdz[q];
     }
     const double K = rep.K;
     unsigned nacc = 0, nrej = 0;
     __syncthreads();
 
     // ---- fine colours ----
-    for (int c = 0; c < sh.ncol; c++) {
-        const int cs = s_cls[c], ce = s_cls[c + 1];
-        for (int sb = cs; sb < ce; sb += NGROUPS) {
-            const int si = sb + group;
-            const bool active = si < ce;
-            const int k = s_ctile[active ? si : cs];   // tile index of the site
+    // A step is up to NGROUPS sites of one fine colour, one per group of G
+    // lanes; the host lists them per tile shape: per step, a barrier flag (set
+    // on a colour's last step) and the tile index of each group's site (-1:
+    // none).  Each group decides its site; then every wave walks its accepted
+    // sites one at a time, all 64 lanes scattering that site's pairs, so a
+    // rejected site costs no scatter instructions.
+    // The site data of a step (dipole, proposal, threshold, coordinates) do
+    // not change while earlier steps run: a site's dipole is written only in
+    // its own step, and sites of one colour never interact.  So step s + 1's
+    // data are loaded with step s's fields, and step s + 2's site indices with
+    // them; after each colour's barrier the field read, the decision and the
+    // atomics remain on the critical path.  Targets in colours already done
+    // are still updated (their fields are never read again).
+    constexpr int SW = NGROUPS + 1;                   // ints per step in the table
+    struct Raw {
+        int k, pc;
+        double ox, oy, oz, qx, qy, qz, thr, Li;
+    };
+    struct Prep {
+        int k, pc;                                    // tile index (>= 0), packed local coordinates
+        int live;                                     // 1: a non-vacant site to decide
+        double dx, dy, dz, qx, qy, qz, thr, kt, Li;
+    };
+    auto load_raw = [&](int kk, Raw &R) __attribute__((always_inline)) {
+        const int k = kk < 0 ? 0 : kk;
+        R.k = kk;
+        R.ox = cpx[k]; R.oy = cpy[k]; R.oz = cpz[k];
+        R.qx = cqx[k]; R.qy = cqy[k]; R.qz = cqz[k];
+        R.thr = cthr[k];
+        R.Li = MIXED ? cl[k] : rep.L0;
+        R.pc = s_coord[k];
+    };
+    auto prep = [&](const Raw &R, Prep &P) __attribute__((always_inline)) {
+        P.k = R.k < 0 ? 0 : R.k;
+        P.pc = R.pc;
+        P.live = (R.k >= 0 && R.Li != 0.0) ? 1 : 0;
+        P.qx = R.qx; P.qy = R.qy; P.qz = R.qz;
+        P.thr = R.thr;
+        P.Li = R.Li;
+        P.dx = R.qx - R.ox; P.dy = R.qy - R.oy; P.dz = R.qz - R.oz;
+        P.kt = (K > 0.0) ? -K * (fabs(R.qx) - fabs(R.ox) + fabs(R.qy) - fabs(R.oy)) : 0.0;
+    };
+    // the prepared values are final before a barrier (the compiler would
+    // otherwise sink their arithmetic past it, onto the next step's critical
+    // path)
+    auto settle = [&](Prep &P) __attribute__((always_inline)) {
+        asm volatile("" : "+v"(P.k), "+v"(P.pc), "+v"(P.live), "+v"(P.dx), "+v"(P.dy), "+v"(P.dz), "+v"(P.qx),
+                     "+v"(P.qy), "+v"(P.qz), "+v"(P.thr), "+v"(P.kt), "+v"(P.Li));
+    };
+    auto readlane_d = [](double v, int l) __attribute__((always_inline)) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)b, l);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    };
+
+    const int nsteps = sh.nsteps;
+    if (nsteps > 0) {
+        Prep cur, nxt;
+        Raw raw;
+        load_raw(s_step[1 + group], raw);
+        int k_next = nsteps > 1 ? s_step[SW + 1 + group] : -1;   // site of step s + 1
+        int flag = s_step[0];
+        prep(raw, cur);
+        for (int st = 0; st < nsteps; st++) {
+            // loads: this step's fields, step s + 1's site data, step s + 2's site
+            const int k = cur.k;
             const double fx = cfx[k], fy = cfy[k], fz = cfz[k];
-            const double ox = cpx[k], oy = cpy[k], oz = cpz[k];
-            const double qx = cqx[k], qy = cqy[k], qz = cqz[k], thr = cthr[k];
-            const double Li = MIXED ? cl[k] : rep.L0;
-            const double dx = qx - ox, dy = qy - oy, dz = qz - oz;
-            double dE = dx * fx + dy * fy + dz * fz;
-            if (K > 0.0) dE += -K * (fabs(qx) - fabs(ox) + fabs(qy) - fabs(oy));
-            const bool acc = active && Li != 0.0 && dE < thr;
-            if (active && part == 0 && Li != 0.0) {
-                if (acc) { cpx[k] = qx; cpy[k] = qy; cpz[k] = qz; nacc++; }
+            const bool more = st + 1 < nsteps;
+            int flag_next = 0, k_next2 = -1;
+            if (more) {
+                load_raw(k_next, raw);
+                flag_next = s_step[(st + 1) * SW];
+                if (st + 2 < nsteps) k_next2 = s_step[(st + 2) * SW + 1 + group];
+            }
+            // decide from the current field (every lane of the group alike)
+            const double dE = fma(cur.dx, fx, fma(cur.dy, fy, cur.dz * fz)) + cur.kt;
+            const bool acc = cur.live && dE < cur.thr;
+            if (cur.live && part == 0) {
+                if (acc) { cpx[k] = cur.qx; cpy[k] = cur.qy; cpz[k] = cur.qz; nacc++; }
                 else nrej++;
             }
-            if (acc) {
-                // f_t += c_t T_r (L_i dp) - J dp [|r| = 1] for tile sites t = i +- r
-                const int pc = s_coord[k];
-                const int lx = pc & 1023, ly = (pc >> 10) & 1023, lz = pc >> 20;
-                const double sc = MIXED ? Li : 1.0;
-                const double mx = sc * dx, my = sc * dy, mz = sc * dz;
+            // the wave's accepted sites, one at a time (lane G g leads group g)
+            unsigned long long bal = __ballot(acc);
+            while (bal) {
+                const int src = __ffsll((long long)bal) - 1;
+                bal &= ~(((G >= 64) ? ~0ull : ((1ull << G) - 1)) << src);
+                const double sdx = readlane_d(cur.dx, src), sdy = readlane_d(cur.dy, src),
+                             sdz = readlane_d(cur.dz, src);
+                const int sk = __builtin_amdgcn_readlane(cur.k, src);
+                const int spc = __builtin_amdgcn_readlane(cur.pc, src);
+                const double sL = MIXED ? readlane_d(cur.Li, src) : 1.0;
+                const int lx = spc & 1023, ly = (spc >> 10) & 1023, lz = spc >> 20;
 #pragma unroll
-                for (int m = 0; m < NPL; m++) {
-                    const double s = pnx[m] * mx + pny[m] * my + pnz[m] * mz;
-                    const double tt = -3.0 * pw[m] * s;
-                    const double vx = pw[m] * mx + tt * pnx[m];
-                    const double vy = pw[m] * my + tt * pny[m];
-                    const double vz = pw[m] * mz + tt * pnz[m];
-                    const double jn = -J * pnn[m];
+                for (int q = 0; q < PPL; q++) {
+                    const double sp = fma(pnx[q], sdx, fma(pny[q], sdy, pnz[q] * sdz));
+                    const double t = pB[q] * sp;
+                    const double ux = fma(pA[q], sdx, t * pnx[q]);
+                    const double uy = fma(pA[q], sdy, t * pny[q]);
+                    const double uz = fma(pA[q], sdz, t * pnz[q]);
 #pragma unroll
-                    for (int sg = -1; sg <= 1; sg += 2) {
-                        const int tx_ = lx + sg * pdx[m], ty_ = ly + sg * pdy[m], tz_ = lz + sg * pdz[m];
+                    for (int h = 0; h < 2; h++) {
+                        const int sg = h ? 1 : -1;
+                        const int tx_ = lx + sg * pdx[q], ty_ = ly + sg * pdy[q], tz_ = lz + sg * pdz[q];
                         if ((unsigned)tx_ < (unsigned)sh.tx && (unsigned)ty_ < (unsigned)sh.ty &&
                             (unsigned)tz_ < (unsigned)sh.tz) {
-                            const int tc = (tx_ * sh.ty + ty_) * sh.tz + tz_;
-                            const double ct = MIXED ? cl[tc] : c_nm;
-                            atomicAdd(&cfx[tc], fma(ct, vx, jn * dx));
-                            atomicAdd(&cfy[tc], fma(ct, vy, jn * dy));
-                            atomicAdd(&cfz[tc], fma(ct, vz, jn * dz));
+                            const int tc = sk + sg * poff[q];
+                            if constexpr (MIXED) {
+                                const double ct = cl[tc] * sL;
+                                atomicAdd(&cfx[tc], fma(ct, ux, pjn[q] * sdx));
+                                atomicAdd(&cfy[tc], fma(ct, uy, pjn[q] * sdy));
+                                atomicAdd(&cfz[tc], fma(ct, uz, pjn[q] * sdz));
+                            } else {
+                                atomicAdd(&cfx[tc], ux);
+                                atomicAdd(&cfy[tc], uy);
+                                atomicAdd(&cfz[tc], uz);
+                            }
                         }
                     }
                 }
             }
+            if (!more) break;
+            prep(raw, nxt);
+            settle(nxt);
+            if (flag) __syncthreads();
+            cur = nxt;
+            flag = flag_next;
+            k_next = k_next2;
         }
         __syncthreads();
     }
 
     // ---- write the tile back ----
     double4 *out = a.lat + rep.base;
-    for (int c = tid; c < ntile; c += kBlock) {
+    for (int c = tid; c < npad; c += kBlock) {
         const int pc = s_coord[c];
+        if (pc < 0) continue;
         const int lx = pc & 1023, ly = (pc >> 10) & 1023, lz = pc >> 20;
         double4 v;
         v.x = cpx[c]; v.y = cpy[c]; v.z = cpz[c];
