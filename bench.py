@@ -127,6 +127,11 @@ def main():
     ap.add_argument("--L", type=int, default=128)
     ap.add_argument("--rc", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the built-in schedule instead of the autotuner's pick")
+    ap.add_argument("--schedule", default=None,
+                    help="fixed schedule 'tx,ty,tz,colouring,kernel' instead of autotuning "
+                         "(profiler runs: keeps trial launches out of the trace)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the headline sweeps (for profiler runs)")
@@ -148,7 +153,16 @@ def main():
     N = rp.N
     n_nb = len(sn.gen_neighbour(rc, L))
     F = flops_per_update(n_nb, 6)
-    ctx = sn.StarryNight(rp)
+    if args.schedule:
+        v = [int(x) for x in args.schedule.split(",")]
+        tun, tuned_rate = sn.TuningParams(tile=tuple(v[:3]), fine_colouring=v[3], kernel=v[4]), None
+    elif args.no_autotune:
+        tun, tuned_rate = None, None
+    else:
+        tun, tuned_rate = sn.autotune(rp, budget_ms=4000, use_cache=False)
+    if tuned_rate is not None:
+        log(f"[bench] autotuner pick: {tun} ({tuned_rate:.3e} site-updates/s in its trial)")
+    ctx = sn.StarryNight(rp, tun)
     ctx.init_lattice_random(0)
     cfg = ctx.sweep_config()
     log(f"[bench] {L}^3 rc={rc} N={N} neighbours={n_nb} F={F} sweep config {cfg}")
@@ -170,7 +184,8 @@ def main():
     mean_ms, nl = ctx.time_sweep_kernel(ksweeps)
     updates_per_launch = N * ksweeps / nl
     achieved = updates_per_launch * F / (mean_ms * 1e-3) / 1e12
-    kname = "field_sweep_kernel" if 1 <= rc <= 5 else "tile_sweep_kernel"
+    field = (tun is None or tun.kernel != 2) and 1 <= rc <= 5
+    kname = "field_sweep_kernel" if field else "tile_sweep_kernel"
     traffic = measured_traffic(kname, L, rc)
     hbm_alg = updates_per_launch * BYTES_PER_UPDATE / (mean_ms * 1e-3) / 1e9
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -192,7 +207,7 @@ def main():
 
     batched = None
     if not args.no_batched:
-        batched = run_batched(sn, rc, log)
+        batched = run_batched(sn, rc, log, autotune=not args.no_autotune)
 
     cpu = None
     if not args.no_cpu:
@@ -209,7 +224,8 @@ def main():
                    "X": L, "Y": L, "Z": L, "DipoleCutOff": rc, "CageStrain": 1.0, "Efield": [0, 0, 0],
                    "T": 300, "neighbours": n_nb, "tiles": cfg["tiles"],
                    "launches_per_sweep": cfg["launches_per_sweep"], "lanes_per_site": cfg["lanes_per_site"],
-                   "fine_colours": cfg["fine_colours"]},
+                   "fine_colours": cfg["fine_colours"],
+                   "schedule": ("autotuned " + str(tun.__dict__)) if tun is not None else "built-in default"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu_all_cores": (rate / cpu["value"]) if cpu else None,
@@ -235,10 +251,11 @@ def batched_replicas(sn, rc):
     return reps
 
 
-def run_batched(sn, rc, log, sweeps=20):
+def run_batched(sn, rc, log, sweeps=20, autotune=True):
     reps = batched_replicas(sn, rc)
     total = sum(r.N for r in reps)
-    ctx = sn.StarryNight(reps)
+    tun = sn.autotune(reps, budget_ms=6000, use_cache=False)[0] if autotune else None
+    ctx = sn.StarryNight(reps, tun)
     for r in range(len(reps)):
         ctx.init_lattice_random(r)
     ctx.MC_sweeps(3)
@@ -250,7 +267,8 @@ def run_batched(sn, rc, log, sweeps=20):
     agg = sweeps * total / dt
     cfg = ctx.sweep_config()
     ctx.close()
-    one = sn.StarryNight(sn.ReplicaParams(32, 32, 32, DipoleCutOff=rc, T=300, seed=7))
+    one_rp = sn.ReplicaParams(32, 32, 32, DipoleCutOff=rc, T=300, seed=7)
+    one = sn.StarryNight(one_rp, sn.autotune(one_rp, budget_ms=3000, use_cache=False)[0] if autotune else None)
     one.init_lattice_random(0)
     one.MC_sweeps(3)
     t0 = time.perf_counter()
@@ -262,7 +280,8 @@ def run_batched(sn, rc, log, sweeps=20):
     log(f"[bench] batched {len(reps)} replicas ({total} sites): {agg:.4e}/s; single 32^3: {single:.4e}/s; "
         f"ratio {agg / single:.1f}")
     return {"replicas": len(reps), "sites_per_sweep": total, "aggregate": agg, "single_32cube": single,
-            "ratio_vs_single_32cube": agg / single, "sweeps": sweeps, "tiles": cfg["tiles"]}
+            "ratio_vs_single_32cube": agg / single, "sweeps": sweeps, "tiles": cfg["tiles"],
+            "schedule": ("autotuned " + str(tun.__dict__)) if tun is not None else "built-in default"}
 
 
 if __name__ == "__main__":

@@ -50,8 +50,8 @@ typedef struct sn_params {
     uint64_t seed;            /* key of the counter-based RNG for this replica */
 } sn_params;
 
-/* Sweep-kernel tuning knobs.  Zero fields mean "choose automatically" (the
- * autotuner's cached choice, else the built-in default). */
+/* Sweep-kernel tuning knobs.  Zero fields mean the built-in default;
+ * sn_autotune picks them by measurement. */
 typedef struct sn_tuning {
     int32_t tile[3];          /* target tile edge per axis (sites), >= DipoleCutOff */
     int32_t lanes_per_site;   /* lanes that share one site's neighbour sum: 2..64 */
@@ -208,6 +208,17 @@ int sn_sweep_config(sn_ctx *ctx, int32_t *tiles, int32_t *launches_per_sweep,
  * around every tile-sweep launch and returns the mean launch duration (ms)
  * and the number of launches timed. */
 int sn_time_sweep_kernel(sn_ctx *ctx, int32_t nsweeps, double *mean_ms, int32_t *launches);
+
+/* Autotuner (no counterpart in the reference): times candidate schedules
+ * (tile edge targets, fine colouring, field-update or gather kernel) for the
+ * given replicas on the current device, each as a real context over a random
+ * lattice, and returns the fastest in *best (site-updates/s in *best_rate).
+ * budget_ms bounds the search (0: try every candidate).  With use_cache != 0
+ * a previous result for the same device, DipoleCutOff and replica sizes is
+ * returned from the cache file ($SN_TUNE_CACHE, else
+ * $HOME/.cache/starrynight_amd/tune.txt) and new results are appended. */
+int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t budget_ms, int32_t use_cache,
+                sn_tuning *best, double *best_rate);
 
 #ifdef __cplusplus
 }

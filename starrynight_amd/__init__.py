@@ -6,9 +6,9 @@ ctypes.  There is no CPU fallback: without the built library every entry point
 raises ImportError.
 """
 from ._lib import EXPORTS, LIB_PATH, SnError, lib  # noqa: F401
-from .lattice import (ReplicaParams, StarryNight, TuningParams, gen_neighbour, plan_schedule,  # noqa: F401
-                      reference_initial_lattice)
+from .lattice import (ReplicaParams, StarryNight, TuningParams, autotune, gen_neighbour,  # noqa: F401
+                      plan_schedule, reference_initial_lattice)
 
 __all__ = ["StarryNight", "ReplicaParams", "TuningParams", "SnError", "gen_neighbour", "plan_schedule",
-           "reference_initial_lattice", "lib",
+           "reference_initial_lattice", "autotune", "lib",
            "LIB_PATH", "EXPORTS"]

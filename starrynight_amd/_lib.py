@@ -25,7 +25,7 @@ EXPORTS = [
     "sn_mc_replay", "sn_counters", "sn_reset_counters", "sn_energy", "sn_gradient", "sn_moment",
     "sn_radial_order",
     "sn_schedule_info", "sn_schedule_order", "sn_stream", "sn_sweep_config",
-    "sn_time_sweep_kernel", "sn_plan_schedule", "sn_reference_initial_lattice",
+    "sn_time_sweep_kernel", "sn_plan_schedule", "sn_autotune", "sn_reference_initial_lattice",
 ]
 
 
@@ -84,6 +84,7 @@ def lib():
         L.sn_stream.argtypes = [vp, P(vp)]
         L.sn_sweep_config.argtypes = [vp, ip, ip, ip, ip]
         L.sn_time_sweep_kernel.argtypes = [vp, C.c_int32, dp, ip]
+        L.sn_autotune.argtypes = [P(Params), C.c_int32, C.c_int32, C.c_int32, P(Tuning), dp]
         L.sn_reference_initial_lattice.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_char_p, C.c_uint32,
                                                    C.c_int32, dp, dp, dp]
         L.sn_plan_schedule.argtypes = [P(Params), C.c_int32, P(Tuning), C.c_int32, P(C.c_int64), ip,

@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 
-SOURCES = ["sn_api.hip", "sn_schedule.cpp", "sn_lattice_init.cpp", "sn_gather_kernels.hip"] + \
+SOURCES = ["sn_api.hip", "sn_schedule.cpp", "sn_lattice_init.cpp", "sn_autotune.cpp", "sn_gather_kernels.hip"] + \
     [f"sn_field_kernels_rc{r}.hip" for r in range(1, 6)]
 
 

@@ -153,7 +153,7 @@ static int ensure_partials(sn_ctx *c, size_t n)
 constexpr int kMaxRegion = 4096;
 // Largest tile: the field kernel gives each lane at most 2 items of 2 sites
 // (3D) or 1 site (2D).
-constexpr int kMaxTile3D = 4 * kBlock, kMaxTile2D = 2 * kBlock;
+constexpr int kMaxTile3D = kFieldMaxTile3D, kMaxTile2D = kFieldMaxTile2D;
 
 static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &t, Schedule &s)
 {

@@ -254,6 +254,10 @@ constexpr int n_pairs()
     return n / 2;
 }
 
+// largest tile (sites) a workgroup takes: each lane gathers at most two
+// items of two sites (3D) or one site (2D)
+constexpr int kFieldMaxTile3D = 4 * kBlock, kFieldMaxTile2D = 2 * kBlock;
+
 struct FieldArgs {
     SweepArgs s;
     const int *steps;         // per shape (step_off): per step, barrier flag + NGROUPS tile indices
@@ -311,12 +315,37 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
     for (int i = tid; i < npad; i += kBlock) s_coord[i] = fa.tile_coord[sh.coord_off + i];
     for (int i = tid; i < sh.nsteps * (NGROUPS + 1); i += kBlock) s_step[i] = fa.steps[sh.step_off + i];
 
-    // ---- stage tile + halo ----
+    // ---- stage tile + halo; the proposals of this lane's tile sites are drawn
+    // while the staging loads are in flight and kept in registers until the
+    // tile-order arrays exist ----
+    const unsigned sweep = a.sweep_ctr[tile.replica];
+    const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
+    const int ntile = sh.tx * sh.ty * sh.tz;
+    constexpr int NPROP = (TWO_D ? kFieldMaxTile2D : kFieldMaxTile3D) / kBlock;
+    struct Prop {
+        double x, y, z, thr;
+        int kp;   // padded tile index, -1: no site
+    };
+    Prop pr[NPROP];
+    auto draw = [&](int i, Prop &P) __attribute__((always_inline)) {
+        const int c = tid + kBlock * i;   // dense tile index (lx * ty + ly) * tz + lz
+        P.kp = -1;
+        P.x = P.y = P.z = P.thr = 0.0;
+        if (c < ntile) {
+            const int lz = c % sh.tz, r_ = c / sh.tz, ly = r_ % sh.ty, lx = r_ / sh.ty;
+            const unsigned gsite = (unsigned)(tile_base + ((long long)lx * rep.Y + ly) * rep.Z + lz);
+            double u;
+            counter_draw(rep.seed, (uint32_t)tile.replica, sweep, gsite, rep.constrain_to_x, P.x, P.y, P.z, u);
+            P.thr = metropolis_threshold(u, rep.beta);
+            P.kp = (lx * TYP + ly) * TZP + lz;
+        }
+    };
     const double4 *lat = a.lat + rep.base;
     {
         int iz = tid % RZ, q = tid / RZ, iy = q % RY, ix = q / RY;
         const int dz_ = kBlock % RZ, q2 = kBlock / RZ, dy_ = q2 % RY, dx_ = q2 / RY;
-        for (int base = tid; base < region; base += 4 * kBlock) {
+        // one batch: 4 loads per lane issued, `between` runs, then the LDS writes
+        auto batch = [&](int base, auto &&between) __attribute__((always_inline)) {
             double4 v[4];
             int idx[4], pidx[4];
 #pragma unroll
@@ -333,6 +362,7 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
                 if (iz >= RZ) { iz -= RZ; iy++; }
                 if (iy >= RY) { iy -= RY; ix++; }
             }
+            between();
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 if (idx[k] < region) {
@@ -340,7 +370,13 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
                     sx[d] = v[k].x; sy[d] = v[k].y; sz[d] = v[k].z;
                     if (MIXED) sl[d] = v[k].w;
                 }
-        }
+        };
+        static_for<0, NPROP - 1>([&](auto ic) __attribute__((always_inline)) {
+            constexpr int i = decltype(ic)::value;
+            batch(tid + i * 4 * kBlock, [&]() __attribute__((always_inline)) { draw(i, pr[i]); });
+        });
+        for (int base = tid + NPROP * 4 * kBlock; base < region; base += 4 * kBlock)
+            batch(base, []() {});
     }
     __syncthreads();
 
@@ -399,19 +435,14 @@ __global__ __launch_bounds__(kBlock) void field_sweep_kernel(FieldArgs fa, int f
     };
     put(o0);
     put(o1);
-    // ---- proposals of every tile site (tile order) ----
-    const unsigned sweep = a.sweep_ctr[tile.replica];
-    const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
-    for (int c = tid; c < npad; c += kBlock) {
-        const int pc = s_coord[c];
-        if (pc < 0) continue;
-        const int lx = pc & 1023, ly = (pc >> 10) & 1023, lz = pc >> 20;
-        unsigned gsite = (unsigned)(tile_base + ((long long)lx * rep.Y + ly) * rep.Z + lz);
-        double px, py, pz, u;
-        counter_draw(rep.seed, (uint32_t)tile.replica, sweep, gsite, rep.constrain_to_x, px, py, pz, u);
-        cqx[c] = px; cqy[c] = py; cqz[c] = pz;
-        cthr[c] = metropolis_threshold(u, rep.beta);
-    }
+    // ---- proposals into the tile-order arrays ----
+#pragma unroll
+    for (int i = 0; i < NPROP; i++)
+        if (pr[i].kp >= 0) {
+            const int c = pr[i].kp;
+            cqx[c] = pr[i].x; cqy[c] = pr[i].y; cqz[c] = pr[i].z;
+            cthr[c] = pr[i].thr;
+        }
 
     // ---- this lane's scatter pairs: pair j = lane + 64 q of the +-r list ----
     // An accepted site i adds to the field of every tile site t = i +- r

@@ -297,3 +297,18 @@ def reference_initial_lattice(X: int, Y: int, Z: int, InitialLattice: str = "ran
     if st != 0:
         raise SnError(st, "invalid initial-lattice request (Prevalence must cover the drawn samples)")
     return out
+
+
+def autotune(replicas, budget_ms: int = 3000, use_cache: bool = True):
+    """sn_autotune: the fastest measured schedule for these replicas on the
+    current device, as (TuningParams, site-updates/s)."""
+    if isinstance(replicas, ReplicaParams):
+        replicas = [replicas]
+    arr = (Params * len(replicas))(*[r.to_c() for r in replicas])
+    t = Tuning()
+    rate = C.c_double(0)
+    st = lib().sn_autotune(arr, len(replicas), int(budget_ms), int(bool(use_cache)), C.byref(t), C.byref(rate))
+    if st != 0:
+        raise SnError(st, "autotune found no runnable schedule")
+    return TuningParams(tile=tuple(t.tile), lanes_per_site=t.lanes_per_site, fine_colouring=t.fine_colouring,
+                        use_graph=t.use_graph, kernel=t.kernel), rate.value
