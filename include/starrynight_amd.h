@@ -59,6 +59,7 @@ typedef struct sn_tuning {
     int32_t use_graph;        /* 0 or 1: replay sweeps from a captured hipGraph; -1: direct launches */
     int32_t kernel;           /* 0 auto, 1 field-update tile kernel, 2 per-phase gather tile kernel
                                  (both visit sites in the same order with the same draws) */
+    int32_t block_threads;    /* gather kernel's threads per tile: 0 or 256, or 512 */
 } sn_tuning;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -43,7 +43,8 @@ class Params(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_int32 * 3), ("lanes_per_site", C.c_int32),
-                ("fine_colouring", C.c_int32), ("use_graph", C.c_int32), ("kernel", C.c_int32)]
+                ("fine_colouring", C.c_int32), ("use_graph", C.c_int32), ("kernel", C.c_int32),
+                ("block_threads", C.c_int32)]
 
 
 _lib = None

@@ -35,6 +35,7 @@ struct sn_ctx {
     Schedule sched;
     sn_tuning tuning{};
     int G = 16, NPL = 4, npairs = 0, noff = 0;
+    int blk = 256;   // gather kernel's threads per workgroup
     long long total_sites = 0;
     size_t lds_bytes = 0;
     hipStream_t stream = nullptr;
@@ -294,17 +295,22 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     // lanes per site and pairs per lane
     int max_class = 1;
     for (const TileShape &s : c->sched.shapes) max_class = std::max(max_class, s.max_class);
+    const int blk = c->tuning.block_threads > 0 ? c->tuning.block_threads : kBlock;
+    if (blk != 256 && blk != 512) return fail(c, SN_ERR_INVALID, "block_threads must be 256 or 512");
+    if (blk != kBlock && c->tuning.kernel != 2)
+        return fail(c, SN_ERR_INVALID, "block_threads applies to the gather kernel (kernel = 2) only");
+    c->blk = blk;
     int G = c->tuning.lanes_per_site;
     if (G <= 0) {
-        G = 32;
-        while (G > 4 && (kBlock / G) < max_class) G >>= 1;
+        G = blk == 512 ? 64 : 32;
+        while (G > 4 && (blk / G) < max_class) G >>= 1;
     }
     int need = (c->npairs + G - 1) / G;
     int NPL = 2;
     while (NPL < need) NPL <<= 1;
     c->G = G; c->NPL = NPL;
     const bool field_ok = pick_field(c->rc, c->two_d, false) != nullptr;
-    if ((c->tuning.kernel == 2 || !field_ok) && !pick_sweep(G, NPL, false))
+    if ((c->tuning.kernel == 2 || !field_ok) && !pick_sweep(G, NPL, false, blk))
         return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
 
     HIPCHK(c, hipGetDevice(&c->device));
@@ -405,7 +411,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if (!c->can_sweep) return SN_OK;
     for (int m = 0; m < 2; m++) {
         const void *f = c->use_field ? (const void *)pick_field(c->rc, c->two_d, m == 1)
-                                     : (const void *)pick_sweep(G, NPL, m == 1);
+                                     : (const void *)pick_sweep(G, NPL, m == 1, c->blk);
         HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need_lds));
     }
     return SN_OK;
@@ -601,7 +607,7 @@ static int launch_coarse(sn_ctx *c, bool mixed, int first, int n)
         hipLaunchKernelGGL(pick_field(c->rc, c->two_d, mixed), dim3(n), dim3(kBlock), field_lds(c, mixed),
                            c->stream, fa, first, n);
     } else {
-        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed), dim3(n), dim3(kBlock), sweep_lds(c, mixed),
+        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed, c->blk), dim3(n), dim3(c->blk), sweep_lds(c, mixed),
                            c->stream, sweep_args(c), first, n);
     }
     HIPCHK(c, hipGetLastError());

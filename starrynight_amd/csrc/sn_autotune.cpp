@@ -9,7 +9,7 @@
 // sn_create and are skipped.
 //
 // Cache: one line per configuration, "key<TAB>t0 t1 t2 lanes colouring graph
-// kernel rate".  The key is the device architecture, DipoleCutOff and the
+// kernel block rate".  The key is the device architecture, DipoleCutOff and the
 // multiset of replica sizes (the lattice contents are not known at tuning
 // time, so mixed and uniform lengths share an entry).  Path: $SN_TUNE_CACHE,
 // else $HOME/.cache/starrynight_amd/tune.txt; no cache when neither is set.
@@ -59,7 +59,7 @@ std::string config_key(const sn_params *p, int n)
         geo[g.str()]++;
     }
     std::ostringstream k;
-    k << "v1|" << name << "|rc" << p[0].dipole_cutoff;
+    k << "v2|" << name << "|rc" << p[0].dipole_cutoff;
     for (auto &kv : geo) k << "|" << kv.first << "*" << kv.second;
     return k.str();
 }
@@ -78,7 +78,7 @@ bool cache_lookup(const std::string &key, sn_tuning *t, double *rate)
         sn_tuning c{};
         double r = 0.0;
         if (v >> c.tile[0] >> c.tile[1] >> c.tile[2] >> c.lanes_per_site >> c.fine_colouring >> c.use_graph >>
-            c.kernel >> r) {
+            c.kernel >> c.block_threads >> r) {
             *t = c;       // the last entry for a key wins
             *rate = r;
             found = true;
@@ -94,7 +94,8 @@ void cache_store(const std::string &key, const sn_tuning &t, double rate)
     make_parent_dirs(path);
     std::ofstream f(path, std::ios::app);
     f << key << '\t' << t.tile[0] << ' ' << t.tile[1] << ' ' << t.tile[2] << ' ' << t.lanes_per_site << ' '
-      << t.fine_colouring << ' ' << t.use_graph << ' ' << t.kernel << ' ' << rate << '\n';
+      << t.fine_colouring << ' ' << t.use_graph << ' ' << t.kernel << ' ' << t.block_threads << ' ' << rate
+      << '\n';
 }
 
 // site-updates/s of one candidate, or < 0 when it cannot run
@@ -151,14 +152,16 @@ extern "C" int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t 
         tiles = {{8, 8, 8}, {8, 8, 6}, {8, 8, 10}, {6, 8, 8}, {6, 6, 6}, {8, 6, 6}, {4, 8, 8}};
     std::vector<int> colourings = (!two_d && rc == 3) ? std::vector<int>{3, 2, 1} : std::vector<int>{2, 1};
     std::vector<sn_tuning> cands;
-    for (int k : {1, 2})
+    // (kernel, threads per tile): field-update; gather with 4 or 8 waves
+    const int kb[3][2] = {{2, 512}, {2, 256}, {1, 0}};
+    for (const auto &v : kb)
         for (int c : colourings)
             for (auto &tl : tiles) {
-                if (k == 2 && c != colourings[0]) continue;   // the gather kernel only with the default colouring
                 sn_tuning t{};
                 t.tile[0] = tl[0]; t.tile[1] = tl[1]; t.tile[2] = tl[2];
                 t.fine_colouring = c;
-                t.kernel = k;
+                t.kernel = v[0];
+                t.block_threads = v[1];
                 cands.push_back(t);
             }
     const double per = std::max(5.0, (double)budget_ms / (double)cands.size() * 0.5);

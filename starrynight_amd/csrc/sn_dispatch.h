@@ -12,8 +12,9 @@ typedef void (*FieldFn)(FieldArgs, int, int);
 // 16 lanes share the scatter of one accepted site in the field-update kernel.
 constexpr int kFieldG = 16;
 
-// per-phase gather kernel for G lanes per site, NPL pairs per lane (or null)
-SweepFn pick_sweep(int G, int NPL, bool mixed);
+// per-phase gather kernel for G lanes per site, NPL pairs per lane and blk
+// threads per workgroup (or null)
+SweepFn pick_sweep(int G, int NPL, bool mixed, int blk);
 // field-update kernel for DipoleCutOff rc (1..5) or null
 FieldFn pick_field(int rc, bool two_d, bool mixed);
 FieldFn pick_field_rc1(bool two_d, bool mixed);

@@ -5,14 +5,16 @@
 
 namespace sn {
 
-SweepFn pick_sweep(int G, int NPL, bool mixed)
+SweepFn pick_sweep(int G, int NPL, bool mixed, int blk)
 {
-#define SN_CASE(g, n)                                                                  \
-    if (G == g && NPL == n) return mixed ? (SweepFn)tile_sweep_kernel<g, n, true>     \
-                                         : (SweepFn)tile_sweep_kernel<g, n, false>;
-    SN_CASE(4, 4) SN_CASE(4, 8) SN_CASE(4, 16) SN_CASE(8, 2) SN_CASE(8, 4) SN_CASE(8, 8) SN_CASE(8, 16)
-    SN_CASE(16, 2) SN_CASE(16, 4) SN_CASE(16, 8) SN_CASE(16, 16) SN_CASE(32, 2) SN_CASE(32, 4)
-    SN_CASE(32, 8)
+#define SN_CASE(g, n, b)                                                                       \
+    if (G == g && NPL == n && blk == b) return mixed ? (SweepFn)tile_sweep_kernel<g, n, true, b>  \
+                                                     : (SweepFn)tile_sweep_kernel<g, n, false, b>;
+    SN_CASE(4, 4, 256) SN_CASE(4, 8, 256) SN_CASE(4, 16, 256) SN_CASE(8, 2, 256) SN_CASE(8, 4, 256)
+    SN_CASE(8, 8, 256) SN_CASE(8, 16, 256) SN_CASE(16, 2, 256) SN_CASE(16, 4, 256) SN_CASE(16, 8, 256)
+    SN_CASE(16, 16, 256) SN_CASE(32, 2, 256) SN_CASE(32, 4, 256) SN_CASE(32, 8, 256)
+    SN_CASE(16, 4, 512) SN_CASE(16, 8, 512) SN_CASE(16, 16, 512) SN_CASE(32, 2, 512) SN_CASE(32, 4, 512)
+    SN_CASE(32, 8, 512) SN_CASE(64, 2, 512) SN_CASE(64, 4, 512)
 #undef SN_CASE
     return nullptr;
 }

@@ -49,10 +49,13 @@ __device__ __forceinline__ double swz_xor16_f64(double v)
 
 // Sum over aligned groups of G lanes, result in every lane of the group:
 // butterfly of row_mirror (i <-> 15-i), row_half_mirror (i <-> 7-i) and
-// quad permutations xor 2, xor 1 (plus a swizzle xor 16 for G = 32).
+// quad permutations xor 2, xor 1, plus a swizzle xor 16 for G >= 32 and a
+// cross-half exchange (lane xor 32) for G = 64.
 template <int G>
 __device__ __forceinline__ double group_sum_dpp(double v)
 {
+    static_assert(G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "G is a power of two <= 64");
+    if (G >= 64) v += __shfl_xor(v, 32, 64);
     if (G >= 32) v += swz_xor16_f64(v);
     if (G >= 16) v += dpp_f64<0x140>(v);
     if (G >= 8) v += dpp_f64<0x141>(v);
@@ -64,8 +67,10 @@ __device__ __forceinline__ double group_sum_dpp(double v)
 // One workgroup updates one tile: stage tile + halo in LDS, draw the
 // proposals of all tile sites, run the fine colours one after another (G lanes
 // share one site's neighbour sum, NPL +-r pairs per lane), write the tile back.
-template <int G, int NPL, bool MIXED>
-__global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int first_tile, int ntiles)
+// BLK threads per workgroup (256 or 512); proposals are drawn in chunks of at
+// most kBlock sites.
+template <int G, int NPL, bool MIXED, int BLK>
+__global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_tile, int ntiles)
 {
     extern __shared__ double smem[];
     const int tid = threadIdx.x;
@@ -92,22 +97,22 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
 
     // ---- shape tables into LDS ----
     const int ntile = sh.tx * sh.ty * sh.tz;
-    for (int i = tid; i < ntile; i += kBlock) s_ri[i] = a.class_ri[sh.sites_off + i];
-    for (int i = tid; i <= sh.ncol; i += kBlock) s_cls[i] = a.class_start[sh.class_off + i];
-    for (int i = tid; i <= sh.nchunks; i += kBlock) s_chunk[i] = a.chunk_col[sh.chunk_off + i];
+    for (int i = tid; i < ntile; i += BLK) s_ri[i] = a.class_ri[sh.sites_off + i];
+    for (int i = tid; i <= sh.ncol; i += BLK) s_cls[i] = a.class_start[sh.class_off + i];
+    for (int i = tid; i <= sh.nchunks; i += BLK) s_chunk[i] = a.chunk_col[sh.chunk_off + i];
 
     // ---- stage tile + halo (periodic wrap), 4 loads in flight per lane ----
     const double4 *lat = a.lat + rep.base;
     {
-        // mixed-radix position of element tid, and the step of kBlock elements
+        // mixed-radix position of element tid, and the step of BLK elements
         int iz = tid % RZ, q = tid / RZ, iy = q % RY, ix = q / RY;
-        const int sz_ = kBlock % RZ, q2 = kBlock / RZ, sy_ = q2 % RY, sx_ = q2 / RY;
-        for (int base = tid; base < region; base += 4 * kBlock) {
+        const int sz_ = BLK % RZ, q2 = BLK / RZ, sy_ = q2 % RY, sx_ = q2 / RY;
+        for (int base = tid; base < region; base += 4 * BLK) {
             double4 v[4];
             int idx[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                idx[k] = base + k * kBlock;
+                idx[k] = base + k * BLK;
                 if (idx[k] < region) {
                     int gx = wrap_once(tile.x0 - hx + ix, rep.X);
                     int gy = wrap_once(tile.y0 - hy + iy, rep.Y);
@@ -131,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
     // ---- this lane's share of the neighbour pairs, held in registers ----
     const int part = tid % G;
     const int group = tid / G;
-    constexpr int ngroups = kBlock / G;
+    constexpr int ngroups = BLK / G;
     int dl[NPL];
     double nx[NPL], ny[NPL], nz[NPL], w[NPL], w3[NPL];
 #pragma unroll
@@ -225,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
 
     // ---- write the tile back ----
     double4 *out = a.lat + rep.base;
-    for (int i = tid; i < ntile; i += kBlock) {
+    for (int i = tid; i < ntile; i += BLK) {
         int lz = i % sh.tz, q = i / sh.tz;
         int ly = q % sh.ty, lx = q / sh.ty;
         int ri = ((lx + hx) * RY + (ly + hy)) * RZ + (lz + hz);
@@ -241,12 +246,12 @@ __global__ __launch_bounds__(kBlock) void tile_sweep_kernel(SweepArgs a, int fir
         ca += __shfl_xor(ca, o, 64);
         cr += __shfl_xor(cr, o, 64);
     }
-    __shared__ unsigned long long s_cnt[2 * (kBlock / 64)];
+    __shared__ unsigned long long s_cnt[2 * (BLK / 64)];
     if ((tid & 63) == 0) { s_cnt[2 * (tid >> 6)] = ca; s_cnt[2 * (tid >> 6) + 1] = cr; }
     __syncthreads();
     if (tid == 0) {
         unsigned long long A_ = 0, R_ = 0;
-        for (int w_ = 0; w_ < kBlock / 64; w_++) { A_ += s_cnt[2 * w_]; R_ += s_cnt[2 * w_ + 1]; }
+        for (int w_ = 0; w_ < BLK / 64; w_++) { A_ += s_cnt[2 * w_]; R_ += s_cnt[2 * w_ + 1]; }
         a.counters[2 * (first_tile + t)] += A_;
         a.counters[2 * (first_tile + t) + 1] += R_;
     }

@@ -58,6 +58,7 @@ class TuningParams:
     fine_colouring: int = 0
     use_graph: int = 0
     kernel: int = 0   # 0 auto, 1 field-update, 2 per-phase gather
+    block_threads: int = 0   # gather kernel: 0 / 256, or 512 threads per tile
 
     def to_c(self) -> Tuning:
         t = Tuning()
@@ -66,6 +67,7 @@ class TuningParams:
         t.fine_colouring = self.fine_colouring
         t.use_graph = self.use_graph
         t.kernel = self.kernel
+        t.block_threads = self.block_threads
         return t
 
 
@@ -311,4 +313,4 @@ def autotune(replicas, budget_ms: int = 3000, use_cache: bool = True):
     if st != 0:
         raise SnError(st, "autotune found no runnable schedule")
     return TuningParams(tile=tuple(t.tile), lanes_per_site=t.lanes_per_site, fine_colouring=t.fine_colouring,
-                        use_graph=t.use_graph, kernel=t.kernel), rate.value
+                        use_graph=t.use_graph, kernel=t.kernel, block_threads=t.block_threads), rate.value

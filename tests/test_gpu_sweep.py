@@ -30,6 +30,12 @@ CASES = {
     "2d_mixed_30x26": (dict(X=30, Y=26, Z=1, T=200, K=0.2), True, {}),
     "rc2_12cube": (dict(X=12, Y=12, Z=12, DipoleCutOff=2, T=300), True, {}),
     "rc5_24cube": (dict(X=24, Y=24, Z=24, DipoleCutOff=5, T=300), False, {}),
+    "16cube_gather_512": (dict(X=16, Y=16, Z=16, T=300, K=0.2), True, dict(kernel=2, block_threads=512)),
+    "20cube_rc4_gather_512": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300), True,
+                              dict(kernel=2, block_threads=512)),
+    "25x25x1_gather_512": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=2, block_threads=512)),
+    "18x20x22_gather_512_cubic": (dict(X=18, Y=20, Z=22, T=350), False,
+                                  dict(kernel=2, block_threads=512, fine_colouring=1)),
 }
 
 
