@@ -46,6 +46,8 @@ struct sn_ctx {
     TileDev *d_tiles = nullptr;
     ShapeDev *d_shapes = nullptr;
     int *d_class_start = nullptr, *d_class_ri = nullptr, *d_chunk_col = nullptr;
+    int *d_gsteps = nullptr, *d_gchunk = nullptr;   // gather kernel's step tables
+    int max_gsteps = 0;
     int *d_field_steps = nullptr, *d_tile_coord = nullptr;
     int max_steps = 0;   // longest step table of a tile shape (field kernel)
     int max_tile_padded = 0;   // largest padded tile (field kernel's tile-order arrays)
@@ -170,7 +172,8 @@ static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &
 static size_t sweep_lds(const sn_ctx *c, bool mixed)
 {
     size_t dbl = (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * kBlock) * sizeof(double);
-    size_t ints = (size_t)(c->max_tile_sites + c->max_cls + 1 + c->max_chunks + 1) * sizeof(int);
+    size_t ints = (size_t)(c->max_tile_sites + c->max_cls + 1 + 2 * (c->max_chunks + 1) +
+                           c->max_gsteps * (c->blk / c->G + 1)) * sizeof(int);
     return dbl + ints;
 }
 
@@ -325,7 +328,8 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     for (const Tile &t : c->sched.tiles) td.push_back({t.replica, t.x0, t.y0, t.z0, t.shape, 0, 0, 0});
     if ((st = upload(c, &c->d_tiles, td))) return st;
     std::vector<ShapeDev> sd;
-    std::vector<int> cstart, cri, chunks, fsteps, tcoord;
+    std::vector<int> cstart, cri, chunks, fsteps, tcoord, gsteps, gchunk;
+    const int NGg = blk / G;   // lane groups of the gather kernel
     constexpr int NG = kBlock / kFieldG;   // lane groups of the field kernel
     const int hz = c->two_d ? 0 : c->rc;
     for (const TileShape &s : c->sched.shapes) {
@@ -374,6 +378,29 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         }
         c->max_steps = std::max(c->max_steps, d.nsteps);
         chunks.insert(chunks.end(), s.chunk_start.begin(), s.chunk_start.end());
+        // gather kernel steps: per proposal chunk, each colour in blocks of NGg
+        // sites; per step a barrier flag (last block of the colour) and NGg
+        // entries (region index | slot in the chunk's proposals << 16, or -1)
+        d.gstep_off = (int)gsteps.size();
+        d.gchunk_off = (int)gchunk.size();
+        int nst = 0;
+        for (int ch = 0; ch < d.nchunks; ch++) {
+            gchunk.push_back(nst);
+            const int c0 = s.chunk_start[ch], c1 = s.chunk_start[ch + 1];
+            const int s0 = s.class_start[c0];
+            for (int col = c0; col < c1; col++) {
+                const int a0 = s.class_start[col], a1 = s.class_start[col + 1];
+                for (int b0 = a0; b0 < a1; b0 += NGg) {
+                    gsteps.push_back(b0 + NGg >= a1 ? 1 : 0);
+                    for (int g = 0; g < NGg; g++)
+                        gsteps.push_back(b0 + g < a1 ? (cri[d.sites_off + b0 + g] | ((b0 + g - s0) << 16)) : -1);
+                    nst++;
+                }
+            }
+        }
+        gchunk.push_back(nst);
+        d.gsteps_n = nst;
+        c->max_gsteps = std::max(c->max_gsteps, nst);
         c->max_tile_sites = std::max(c->max_tile_sites, s.tx * s.ty * s.tz);
         const int RZp = c->two_d ? RZ : (RZ | 1);
         c->field_region = std::max(c->field_region, (s.tx + 2 * c->rc) * RY * RZp);
@@ -387,6 +414,8 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if ((st = upload(c, &c->d_field_steps, fsteps))) return st;
     if ((st = upload(c, &c->d_tile_coord, tcoord))) return st;
     if ((st = upload(c, &c->d_chunk_col, chunks))) return st;
+    if ((st = upload(c, &c->d_gsteps, gsteps))) return st;
+    if ((st = upload(c, &c->d_gchunk, gchunk))) return st;
     if ((st = upload(c, &c->d_pairs, hp))) return st;
     if ((st = upload(c, &c->d_offs, ho))) return st;
     std::vector<unsigned> zeros_u(c->R, 0u);
@@ -423,7 +452,8 @@ int sn_destroy(sn_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph(c);
     void *bufs[] = {c->d_lat, c->d_reps, c->d_tiles, c->d_shapes, c->d_class_start, c->d_class_ri,
-                    c->d_field_steps, c->d_tile_coord, c->d_chunk_col, c->d_pairs, c->d_offs, c->d_sweep,
+                    c->d_field_steps, c->d_tile_coord, c->d_chunk_col, c->d_gsteps, c->d_gchunk, c->d_pairs,
+                    c->d_offs, c->d_sweep,
                     c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -583,6 +613,8 @@ static SweepArgs sweep_args(sn_ctx *c)
     SweepArgs a;
     a.lat = c->d_lat; a.reps = c->d_reps; a.tiles = c->d_tiles; a.shapes = c->d_shapes;
     a.class_start = c->d_class_start; a.class_ri = c->d_class_ri; a.chunk_col = c->d_chunk_col;
+    a.gsteps = c->d_gsteps; a.gchunk = c->d_gchunk;
+    a.max_gsteps = c->max_gsteps; a.max_chunks = c->max_chunks;
     a.pairs = c->d_pairs; a.sweep_ctr = c->d_sweep; a.counters = c->d_counters;
     a.npairs = c->npairs; a.rc = c->rc; a.two_d = c->two_d ? 1 : 0;
     a.max_region = c->sched.max_region;

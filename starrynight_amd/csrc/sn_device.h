@@ -25,7 +25,7 @@ struct ShapeDev {
     int nchunks, class_off, chunk_off, sites_off;
     int nsteps, step_off;     // field kernel's step table (see sn_sweep_field.h)
     int typ, tzp, coord_off;  // field kernel's padded tile-order strides and coordinate table
-    int pad_;
+    int gstep_off, gchunk_off, gsteps_n;   // gather kernel's step table (see sn_kernels.h)
 };
 
 // One +-r neighbour pair (sweep kernel) or one offset (gather kernels).

@@ -24,17 +24,23 @@ struct SweepArgs {
     const PairDev *pairs;
     const unsigned *sweep_ctr;
     unsigned long long *counters;  // 2 per tile: accepted, rejected (accumulated over sweeps)
+    const int *gsteps;        // per shape (gstep_off): per step, barrier flag + BLK / G entries
+                              // (region index | proposal slot << 16, or -1)
+    const int *gchunk;        // per shape (gchunk_off): first step of each proposal chunk (nchunks + 1)
     int npairs, rc, two_d, max_region;
     int max_tile_sites, max_cls;   // sizes of the LDS copies of the shape tables
+    int max_gsteps, max_chunks;
 };
 
 // DPP lane exchange of a double (two 32-bit moves); CTRL is a dpp_ctrl code.
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v)
 {
+    // every control used here permutes lanes within a row, so each lane has a
+    // source and no old value is needed
     int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 
@@ -67,6 +73,8 @@ __device__ __forceinline__ double group_sum_dpp(double v)
 // One workgroup updates one tile: stage tile + halo in LDS, draw the
 // proposals of all tile sites, run the fine colours one after another (G lanes
 // share one site's neighbour sum, NPL +-r pairs per lane), write the tile back.
+__device__ __forceinline__ int s_gchunk_total(const ShapeDev &sh) { return sh.gsteps_n; }
+
 // BLK threads per workgroup (256 or 512); proposals are drawn in chunks of at
 // most kBlock sites.
 template <int G, int NPL, bool MIXED, int BLK>
@@ -94,12 +102,19 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     int *s_ri = (int *)(prop + 4 * kBlock);                // region index of class sites
     int *s_cls = s_ri + a.max_tile_sites;                  // class starts, then chunk colours
     int *s_chunk = s_cls + a.max_cls + 1;
+    int *s_gchunk = s_chunk + a.max_chunks + 1;            // first step of each chunk
+    int *s_gstep = s_gchunk + a.max_chunks + 1;            // step table
+    constexpr int SW = BLK / G + 1;                        // ints per step
 
     // ---- shape tables into LDS ----
     const int ntile = sh.tx * sh.ty * sh.tz;
     for (int i = tid; i < ntile; i += BLK) s_ri[i] = a.class_ri[sh.sites_off + i];
     for (int i = tid; i <= sh.ncol; i += BLK) s_cls[i] = a.class_start[sh.class_off + i];
-    for (int i = tid; i <= sh.nchunks; i += BLK) s_chunk[i] = a.chunk_col[sh.chunk_off + i];
+    for (int i = tid; i <= sh.nchunks; i += BLK) {
+        s_chunk[i] = a.chunk_col[sh.chunk_off + i];
+        s_gchunk[i] = a.gchunk[sh.gchunk_off + i];
+    }
+    for (int i = tid; i < s_gchunk_total(sh) * SW; i += BLK) s_gstep[i] = a.gsteps[sh.gstep_off + i];
 
     // ---- stage tile + halo (periodic wrap), 4 loads in flight per lane ----
     const double4 *lat = a.lat + rep.base;
@@ -136,7 +151,6 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     // ---- this lane's share of the neighbour pairs, held in registers ----
     const int part = tid % G;
     const int group = tid / G;
-    constexpr int ngroups = BLK / G;
     int dl[NPL];
     double nx[NPL], ny[NPL], nz[NPL], w[NPL], w3[NPL];
 #pragma unroll
@@ -159,6 +173,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     unsigned nacc = 0, nrej = 0;
     __syncthreads();
 
+    const int ri_origin = (hx * RY + hy) * RZ + hz;   // tile-local (0, 0, 0)
     for (int ch = 0; ch < sh.nchunks; ch++) {
         const int c0 = s_chunk[ch], c1 = s_chunk[ch + 1];
         const int s0 = s_cls[c0], s1 = s_cls[c1];
@@ -176,13 +191,19 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
             prop[3 * kBlock + tid] = metropolis_threshold(u, rep.beta);
         }
         __syncthreads();
-        for (int c = c0; c < c1; c++) {
-            const int cs = s_cls[c], ce = s_cls[c + 1];
-            for (int sb = cs; sb < ce; sb += ngroups) {
-                const int si = sb + group;
-                const bool active = si < ce;
-                const int ri = s_ri[active ? si : cs];
-
+        // this chunk's steps: every group decides one site per step; a step
+        // holds sites of one colour and a colour's last step ends with the
+        // barrier.  The next step's entry and flag load with this step's data.
+        const int st0 = s_gchunk[ch], st1 = s_gchunk[ch + 1];
+        int e = st0 < st1 ? s_gstep[st0 * SW + 1 + group] : -1;
+        int flag = st0 < st1 ? s_gstep[st0 * SW] : 0;
+        for (int st = st0; st < st1; st++) {
+            const bool active = e >= 0;
+            const int ri = active ? (e & 0xFFFF) : ri_origin;   // any tile site keeps the reads in range
+            const int pq = active ? (e >> 16) : 0;
+            const int e_next = st + 1 < st1 ? s_gstep[(st + 1) * SW + 1 + group] : -1;
+            const int flag_next = st + 1 < st1 ? s_gstep[(st + 1) * SW] : 0;
+            {
                 double hx_ = 0.0, hy_ = 0.0, hz_ = 0.0;
                 double gx_ = 0.0, gy_ = 0.0, gz_ = 0.0;
 #pragma unroll
@@ -205,7 +226,6 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                     hz_ += w[m] * mz + ts * nz[m];
                     if (m == 0) { gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz; }
                 }
-                const int pq = active ? si - s0 : 0;
                 const double px = prop[pq], py = prop[kBlock + pq], pz = prop[2 * kBlock + pq];
                 const double thr = prop[3 * kBlock + pq];
                 const double ox = sx[ri], oy = sy[ri], oz = sz[ri];
@@ -224,7 +244,9 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                     } else nrej++;
                 }
             }
-            __syncthreads();
+            if (flag) __syncthreads();
+            e = e_next;
+            flag = flag_next;
         }
     }
 
