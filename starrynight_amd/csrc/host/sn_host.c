@@ -15,6 +15,14 @@
  *   Seed = n                        RNG key (default 0xDEADBEEF + T)
  *   Tile = [8, 8, 8]                tile edge the schedule aims for
  *   SaveLattice = "file"            final lattice of replica 0, double[N][4]
+ *   Autotune = true                 pick the sweep schedule by measurement (sn_autotune)
+ *
+ * CalculateRadialOrderParameter writes the reference's RDF files
+ * (src/starrynight-main.c:39, 58, 75-76): rows appended to "rdf.dat" after
+ * initialisation and to "T_%04d_%d_%03d-RDF.dat" (T, (int) CageStrain,
+ * megastep) after every megastep, in the format of
+ * src/starrynight-analysis.c:582-594; with several replicas the file names
+ * carry "_r<replica>" before the extension.
  *
  * usage: sn_host [-c config] [T [CageStrain]]
  */
@@ -27,6 +35,34 @@
 
 #include "sn_config.h"
 #include "starrynight_amd.h"
+
+/* radial_order_parameter's file rows (src/starrynight-analysis.c:582-594):
+ * header, then "r2 r FE AFE count T" for every occupied bin, then a blank line. */
+static int append_rdf(sn_ctx *ctx, int replica, const char *name, int T)
+{
+    double fe[81], afe[81];
+    int64_t cnt[81];
+    int st = sn_radial_order(ctx, replica, fe, afe, cnt);
+    if (st != SN_OK) return st;
+    FILE *f = fopen(name, "a");
+    if (!f) return SN_ERR_INVALID;
+    fprintf(f, "# r^2 r orientational_FE_correlation[r^2] orientational_AFE_correlation[r^2] "
+               "orientational_count[r^2] T\n");
+    for (int i = 0; i < 81; i++)
+        if (cnt[i] > 0)
+            fprintf(f, "%d %f %f %f %lld %d\n", i, sqrt((double)i), fe[i], afe[i], (long long)cnt[i], T);
+    fprintf(f, "\n");
+    fclose(f);
+    return SN_OK;
+}
+
+/* the reference's file name, with "_r<replica>" before the extension when
+ * several replicas run */
+static void rdf_name(char *out, size_t n, const char *base, const char *ext, int replica, int R)
+{
+    if (R > 1) snprintf(out, n, "%s_r%03d%s", base, replica, ext);
+    else snprintf(out, n, "%s%s", base, ext);
+}
 
 static double now_s(void)
 {
@@ -148,6 +184,18 @@ int main(int argc, char **argv)
     const snc_node *tile = snc_lookup(&cf, "Tile");
     if (tile && snc_length(tile) == 3)
         for (int k = 0; k < 3; k++) tun.tile[k] = tile->kids[k]->type == SNC_INT ? (int)tile->kids[k]->ival : 0;
+    int autotune = 0;
+    snc_lookup_bool(&cf, "Autotune", &autotune);
+    if (autotune) {
+        double rate = 0.0;
+        int ast = sn_autotune(ps, R, 3000, 1, &tun, &rate);
+        if (ast != SN_OK) { fprintf(stderr, "sn_autotune failed: %s\n", sn_status_string(ast)); return 1; }
+        fprintf(stderr, "Autotune: tile %d %d %d, colouring %d, kernel %d, %d threads: %.1f MHz in trial\n",
+                tun.tile[0], tun.tile[1], tun.tile[2], tun.fine_colouring, tun.kernel, tun.block_threads,
+                1e-6 * rate);
+    }
+    int rdf = 0;
+    snc_lookup_bool(&cf, "CalculateRadialOrderParameter", &rdf);
 
     fprintf(stderr, "Lattice %dx%dx%d, DipoleCutOff %d, CageStrain %f, Efield (%f %f %f), K %f, %d replica(s)\n",
             r.X, r.Y, r.Z, r.rc, r.J, r.E[0], r.E[1], r.E[2], r.K, R);
@@ -166,6 +214,12 @@ int main(int argc, char **argv)
         CHECK(ctx, sn_reference_initial_lattice(r.X, r.Y, r.Z, r.initial, mt_seed, r.nspec, r.len, r.prev, lat));
         CHECK(ctx, sn_set_lattice(ctx, i, lat));
     }
+    if (rdf)   /* analysis_initial (src/starrynight-main.c:39) */
+        for (int i = 0; i < R; i++) {
+            char name[256];
+            rdf_name(name, sizeof(name), "rdf", ".dat", i, R);
+            CHECK(ctx, append_rdf(ctx, i, name, (int)Ts[i % nT]));
+        }
     int sweeps = (int)lround(r.moves);
     if (fabs(r.moves - sweeps) > 1e-9)
         fprintf(stderr, "MCMoves = %g: the GPU runs whole sweeps, using %d per megastep\n", r.moves, sweeps);
@@ -220,6 +274,12 @@ int main(int argc, char **argv)
             CHECK(ctx, sn_energy(ctx, i, &E, NULL));
             printf("%d %d %g %f %f %f\n", m, i, Ts[i % nT], E / N,
                    sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]) / N, M[0] / N);
+            if (rdf) {   /* analysis_midpoint (src/starrynight-main.c:58, 75-76) */
+                char base[128], name[256];
+                snprintf(base, sizeof(base), "T_%04d_%d_%03d-RDF", (int)Ts[i % nT], (int)r.J, m);
+                rdf_name(name, sizeof(name), base, ".dat", i, R);
+                CHECK(ctx, append_rdf(ctx, i, name, (int)Ts[i % nT]));
+            }
         }
         fflush(stdout);
         fprintf(stderr, "MC Moves (per second): %f MHz\n", 1e-6 * (double)N * sweeps * R / (t1 - t0));

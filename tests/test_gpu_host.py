@@ -1,0 +1,107 @@
+"""The C host (sn_host) end to end: reads a starrynight.cfg, builds the
+reference's initial lattice, runs the sweeps on the GPU and writes the
+reference's log lines and RDF files."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.helpers import print_case
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "starrynight_amd", "sn_host")
+
+
+def run_host(tmp_path, cfg, *args):
+    (tmp_path / "starrynight.cfg").write_text(cfg)
+    p = subprocess.run([HOST, *args], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(p.stderr[-2000:], flush=True)
+    assert p.returncode == 0, p.stderr
+    return p.stdout, p.stderr
+
+
+def read_rdf(path):
+    blocks, cur = [], []
+    for line in open(path):
+        if line.startswith("#"):
+            continue
+        if not line.strip():
+            if cur:
+                blocks.append(np.array(cur))
+            cur = []
+            continue
+        cur.append([float(v) for v in line.split()])
+    if cur:
+        blocks.append(np.array(cur))
+    return blocks
+
+
+CFG = """
+T: 250
+X = 24
+Y = 24
+Z = 24
+InitialLattice = "random"
+Efield: { x = 0.0; y = 0.0; z = 0.02; }
+DipoleCutOff: 3
+ConstrainToX: false
+K: 0.0
+Dipoles    = [ 1.0, 0.5 ];
+Prevalence = [ 0.7, 0.3 ];
+CageStrain = 2.0
+MCEqmSteps: 1
+MCMegaSteps: 3
+MCMoves: 4.0
+CalculateRadialOrderParameter: true
+SaveLattice = "final.bin";
+"""
+
+
+def test_host_reference_run(sn, oracle, tmp_path):
+    print_case("sn_host", cfg="24^3 T=250 mixed")
+    if not os.path.exists(HOST):
+        pytest.skip("sn_host not built")
+    out, err = run_host(tmp_path, CFG)
+    assert err.count("MC Moves (per second):") == 3 and "ACCEPT:" in err
+    rows = [l.split() for l in out.splitlines() if l and not l.startswith("#")]
+    assert len(rows) == 3 and all(r[1] == "0" and float(r[2]) == 250 for r in rows)
+    # the initial RDF is that of the reference's initial lattice (MT19937 seeded
+    # 0xDEADBEEF + T, random dipoles, then the solid solution), checked with the oracle
+    lat32, _ = oracle.random_lattice_f32(24, 24, 24, 0xDEADBEEF + 250, lengths=(1.0, 0.5),
+                                         prevalence=(0.7, 0.3))
+    fe, afe, cnt = oracle.Oracle(24, 24, 24, 3).radial_order(lat32.astype(np.float64))
+    b0 = read_rdf(tmp_path / "rdf.dat")[0]
+    occ = np.nonzero(cnt)[0]
+    assert np.array_equal(b0[:, 0].astype(int), occ)
+    assert np.array_equal(b0[:, 4].astype(np.int64), cnt[occ])
+    assert np.abs(b0[:, 2] - fe[occ]).max() <= 1.5e-6 and np.abs(b0[:, 3] - afe[occ]).max() <= 1.5e-6
+    assert np.all(b0[:, 5] == 250)
+    for m in range(3):
+        assert (tmp_path / f"T_0250_2_{m:03d}-RDF.dat").exists()
+    # final lattice: unit dipoles, lengths from the species table, energy line consistent
+    fin = np.fromfile(tmp_path / "final.bin", dtype=np.float64).reshape(-1, 4)
+    assert fin.shape == (24 ** 3, 4)
+    # accepted moves write FP64 unit vectors; never-accepted sites keep the
+    # reference's float initial dipoles (|p| = 1 to float rounding)
+    assert np.abs(np.linalg.norm(fin[:, :3], axis=1) - 1).max() < 3e-7
+    assert np.array_equal(fin[:, 3], lat32[:, 3].astype(np.float64))
+    e_last = float(rows[-1][3])
+    o = oracle.Oracle(24, 24, 24, 3, J=2.0, E=(0.0, 0.0, float(np.float32(0.02))))
+    assert abs(o.energy(fin)[0] / 24 ** 3 - e_last) < 2e-6
+
+
+def test_host_batched_schedule(sn, tmp_path):
+    print_case("sn_host", cfg="batched temperatures")
+    if not os.path.exists(HOST):
+        pytest.skip("sn_host not built")
+    cfg = CFG.replace("SaveLattice = \"final.bin\";", "TemperatureSchedule = [ 100, 300, 500 ];\nReplicas = 2;")
+    out, err = run_host(tmp_path, cfg)
+    rows = [l.split() for l in out.splitlines() if l and not l.startswith("#")]
+    assert len(rows) == 3 * 6
+    assert sorted({float(r[2]) for r in rows}) == [100.0, 300.0, 500.0]
+    for i in range(6):
+        assert (tmp_path / f"rdf_r{i:03d}.dat").exists()
+    assert "ACCEPT:" in err
