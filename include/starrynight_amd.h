@@ -180,6 +180,18 @@ int sn_moment(sn_ctx *ctx, int32_t replica, double M[3]);
  * (the reference indexes out of its array there). */
 int sn_radial_order(sn_ctx *ctx, int32_t replica, double fe[81], double afe[81], int64_t count[81]);
 
+/* dipole_potential (src/starrynight-analysis.c:65-94) at every site of replica
+ * r: pot[i] = sum_{0 < |r| <= 6} L_j (p_j . r) / |r|^3, j = i + r, in FP64
+ * (the reference sums float terms in double).  pot: double[N], site order as
+ * the lattice.  The reference writes these values as XYZ maps and .cube files
+ * (:264-308); the caller formats them. */
+int sn_dipole_potential(sn_ctx *ctx, int32_t replica, double *pot);
+/* dipole_electricfield (src/starrynight-analysis.c:393-457) at every site, as
+ * a vector: E[i] = sum_{0 < |r| <= cutoff} (3 n (n . p_j) - p_j) / |r|^3 - p_i / 3
+ * (lengths not applied, as in the reference, which reports |E[i]|).
+ * E: double[N][3]; cutoff in 1..9. */
+int sn_dipole_efield(sn_ctx *ctx, int32_t replica, int32_t cutoff, double *E);
+
 /* ---- schedule introspection (pure host logic, no device work) ----------- */
 
 /* Number of sites N of replica r and number of sequential phases per sweep. */

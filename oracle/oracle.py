@@ -95,6 +95,8 @@ def lib():
         L.or_local_energy_f64.restype = C.c_double
         L.or_moment_sum.argtypes = [dp, C.c_int64, dp]
         L.or_radial_order_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, P(C.c_int64)]
+        L.or_dipole_potential_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+        L.or_dipole_efield_f64.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -228,6 +230,16 @@ class Oracle:
         p = np.ascontiguousarray(p, np.float64)
         return lib().or_local_energy_f64(C.byref(self.m), self.nb, self.nnb, _p(lat64, C.c_double),
                                          x, y, z, _p(p, C.c_double))
+
+    def dipole_potential(self, lat64):
+        pot = np.empty(self.N, np.float64)
+        lib().or_dipole_potential_f64(self.X, self.Y, self.Z, _p(lat64, C.c_double), _p(pot, C.c_double))
+        return pot
+
+    def dipole_efield(self, lat64, cutoff):
+        E = np.empty((self.N, 3), np.float64)
+        lib().or_dipole_efield_f64(self.X, self.Y, self.Z, int(cutoff), _p(lat64, C.c_double), _p(E, C.c_double))
+        return E
 
     def radial_order(self, lat64):
         fe = np.empty(81); afe = np.empty(81); cnt = np.empty(81, np.int64)

@@ -558,3 +558,55 @@ void or_radial_order_f64(int X, int Y, int Z, const double *lat, double fe[81], 
     for (int i = 0; i < 81; i++)
         if (count[i] > 0) { fe[i] /= (double)count[i]; afe[i] /= (double)count[i]; }
 }
+
+/* dipole_potential (src/starrynight-analysis.c:65-94), FP64.  Offsets in
+ * [-6,6]^3 with 0 < |r| <= 6; periodic wrap (the reference's (X+x+dx)%X is
+ * the same site for X >= 6). */
+void or_dipole_potential_f64(int X, int Y, int Z, const double *lat, double *pot)
+{
+    const int MAX = 6;
+    or_model m = {X, Y, Z, 0, 0, {0, 0, 0}, 0, 0, 0};
+    for (int x = 0; x < X; x++)
+        for (int y = 0; y < Y; y++)
+            for (int z = 0; z < Z; z++) {
+                double acc = 0.0;
+                for (int dx = -MAX; dx <= MAX; dx++)
+                    for (int dy = -MAX; dy <= MAX; dy++)
+                        for (int dz = -MAX; dz <= MAX; dz++) {
+                            int d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 == 0 || d2 > MAX * MAX) continue;
+                            const double *b = lat + 4 * site_index(&m, (x + dx + 9 * X) % X,
+                                                                   (y + dy + 9 * Y) % Y, (z + dz + 9 * Z) % Z);
+                            double d = sqrt((double)d2);
+                            acc += b[3] * (b[0] * dx + b[1] * dy + b[2] * dz) / (d * d * d);
+                        }
+                pot[site_index(&m, x, y, z)] = acc;
+            }
+}
+
+/* dipole_electricfield (src/starrynight-analysis.c:393-457), FP64 vector. */
+void or_dipole_efield_f64(int X, int Y, int Z, int cutoff, const double *lat, double *E)
+{
+    or_model m = {X, Y, Z, 0, 0, {0, 0, 0}, 0, 0, 0};
+    for (int x = 0; x < X; x++)
+        for (int y = 0; y < Y; y++)
+            for (int z = 0; z < Z; z++) {
+                double e[3] = {0, 0, 0};
+                for (int dx = -cutoff; dx <= cutoff; dx++)
+                    for (int dy = -cutoff; dy <= cutoff; dy++)
+                        for (int dz = -cutoff; dz <= cutoff; dz++) {
+                            int d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 == 0 || d2 > cutoff * cutoff) continue;
+                            const double *b = lat + 4 * site_index(&m, (x + dx + 9 * X) % X,
+                                                                   (y + dy + 9 * Y) % Y, (z + dz + 9 * Z) % Z);
+                            double d = sqrt((double)d2);
+                            double n[3] = {dx / d, dy / d, dz / d};
+                            double radial = dotd(n, b);
+                            double d3 = d * d * d;
+                            for (int k = 0; k < 3; k++) e[k] += (3.0 * n[k] * radial - b[k]) / d3;
+                        }
+                const double *a = lat + 4 * site_index(&m, x, y, z);
+                double *o = E + 3 * site_index(&m, x, y, z);
+                for (int k = 0; k < 3; k++) o[k] = e[k] - a[k] / 3.0;
+            }
+}

@@ -122,6 +122,15 @@ void or_moment_sum(const double *lat, int64_t N, double M[3]);
 /* radial_order_parameter: bins r^2 = 0..80 (bin 81 never reported). */
 void or_radial_order_f64(int X, int Y, int Z, const double *lat, double fe[81], double afe[81],
                          int64_t count[81]);
+/* dipole_potential (src/starrynight-analysis.c:65-94) at every site, FP64:
+ * pot_i = sum_{0 < |r| <= 6} L_j (p_j . r) / |r|^3, j = i + r (the dz range
+ * runs when Z == 1, onto the same plane). */
+void or_dipole_potential_f64(int X, int Y, int Z, const double *lat, double *pot);
+/* dipole_electricfield (src/starrynight-analysis.c:393-457) at every site,
+ * FP64, as a vector: E_i = sum_{0 < |r| <= cutoff} (3 n (n . p_j) - p_j) / |r|^3
+ * - p_i / 3 (lengths are not applied, as in the reference); the reference
+ * returns |E_i|. */
+void or_dipole_efield_f64(int X, int Y, int Z, int cutoff, const double *lat, double *E);
 
 #ifdef __cplusplus
 }

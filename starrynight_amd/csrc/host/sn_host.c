@@ -56,6 +56,81 @@ static int append_rdf(sn_ctx *ctx, int replica, const char *name, int T)
     return SN_OK;
 }
 
+/* "x y z value" rows of lattice_potential_XYZ / lattice_Efield_XYZ
+ * (src/starrynight-analysis.c:264-276, 466-479) */
+static int write_xyz(const char *name, int X, int Y, int Z, const double *v)
+{
+    FILE *f = fopen(name, "w");
+    if (!f) return SN_ERR_INVALID;
+    for (int x = 0; x < X; x++)
+        for (int y = 0; y < Y; y++)
+            for (int z = 0; z < Z; z++) fprintf(f, "%d %d %d %f\n", x, y, z, v[((long long)x * Y + y) * Z + z]);
+    fclose(f);
+    return SN_OK;
+}
+
+/* lattice_potential_cube's Gaussian cube file (src/starrynight-analysis.c:280-308) */
+static int write_cube(const char *name, int X, int Y, int Z, const double *v)
+{
+    FILE *f = fopen(name, "w");
+    if (!f) return SN_ERR_INVALID;
+    fprintf(f, "Starrynight Cube file: %d %d %d\n\n", X, Y, Z);
+    fprintf(f, "1 0.0 0.0 0.0\n");
+    fprintf(f, "%d 1.0 0.0 0.0\n", X);
+    fprintf(f, "%d 0.0 1.0 0.0\n", Y);
+    fprintf(f, "%d 0.0 0.0 1.0\n", Z);
+    fprintf(f, "1 0.0 0.0 0.0 0.0 0.0\n");
+    for (int x = 0; x < X; x++)
+        for (int y = 0; y < Y; y++) {
+            for (int z = 0; z < Z; z++) {
+                fprintf(f, "%g ", v[((long long)x * Y + y) * Z + z]);
+                if (z % 6 == 5) fprintf(f, "\n");
+            }
+            fprintf(f, "\n");
+        }
+    fclose(f);
+    return SN_OK;
+}
+
+/* potential / field maps of one replica after a megastep (or equilibration):
+ * prefix as the reference's analysis_midpoint (src/starrynight-main.c:85-92) */
+static int write_maps(sn_ctx *ctx, int replica, int R, const char *prefix, int X, int Y, int Z, int potential,
+                      int cube, int efield, double *buf)
+{
+    char base[160], name[256];
+    int st;
+    if (potential || cube) {
+        if ((st = sn_dipole_potential(ctx, replica, buf))) return st;
+        if (potential) {
+            snprintf(base, sizeof(base), "%s_potential", prefix);
+            if (R > 1) snprintf(name, sizeof(name), "%s_r%03d.xyz", base, replica);
+            else snprintf(name, sizeof(name), "%s.xyz", base);
+            if ((st = write_xyz(name, X, Y, Z, buf))) return st;
+        }
+        if (cube) {
+            snprintf(base, sizeof(base), "%s_potential", prefix);
+            if (R > 1) snprintf(name, sizeof(name), "%s_r%03d.cube", base, replica);
+            else snprintf(name, sizeof(name), "%s.cube", base);
+            if ((st = write_cube(name, X, Y, Z, buf))) return st;
+        }
+    }
+    if (efield) {
+        /* lattice_Efield_XYZ reports |E| with cutoff 4 (src/starrynight-analysis.c:477) */
+        const long long N = (long long)X * Y * Z;
+        double *E = (double *)malloc(sizeof(double) * 3 * N);
+        if (!E) return SN_ERR_NOMEM;
+        if ((st = sn_dipole_efield(ctx, replica, 4, E))) { free(E); return st; }
+        for (long long i = 0; i < N; i++)
+            buf[i] = sqrt(E[3 * i] * E[3 * i] + E[3 * i + 1] * E[3 * i + 1] + E[3 * i + 2] * E[3 * i + 2]);
+        free(E);
+        snprintf(base, sizeof(base), "%s_efield", prefix);
+        if (R > 1) snprintf(name, sizeof(name), "%s_r%03d.xyz", base, replica);
+        else snprintf(name, sizeof(name), "%s.xyz", base);
+        if ((st = write_xyz(name, X, Y, Z, buf))) return st;
+    }
+    return SN_OK;
+}
+
 /* the reference's file name, with "_r<replica>" before the extension when
  * several replicas run */
 static void rdf_name(char *out, size_t n, const char *base, const char *ext, int replica, int R)
@@ -194,8 +269,11 @@ int main(int argc, char **argv)
                 tun.tile[0], tun.tile[1], tun.tile[2], tun.fine_colouring, tun.kernel, tun.block_threads,
                 1e-6 * rate);
     }
-    int rdf = 0;
+    int rdf = 0, want_pot = 0, want_cube = 0, want_efield = 0;
     snc_lookup_bool(&cf, "CalculateRadialOrderParameter", &rdf);
+    snc_lookup_bool(&cf, "CalculatePotential", &want_pot);
+    snc_lookup_bool(&cf, "SavePotentialCube", &want_cube);
+    snc_lookup_bool(&cf, "CalculateEfield", &want_efield);
 
     fprintf(stderr, "Lattice %dx%dx%d, DipoleCutOff %d, CageStrain %f, Efield (%f %f %f), K %f, %d replica(s)\n",
             r.X, r.Y, r.Z, r.rc, r.J, r.E[0], r.E[1], r.E[2], r.K, R);
@@ -228,6 +306,8 @@ int main(int argc, char **argv)
     fprintf(stderr, "Equilibriation MC moves... %e\n", (double)N * sweeps * r.eqm);
     for (int i = 0; i < r.eqm; i++) CHECK(ctx, sn_mc_sweeps(ctx, sweeps));
     CHECK(ctx, sn_sync(ctx));
+    if (want_efield)   /* "equilib_lattice_efield.xyz" (src/starrynight-main.c:225) */
+        for (int i = 0; i < R; i++) CHECK(ctx, write_maps(ctx, i, R, "equilib_lattice", r.X, r.Y, r.Z, 0, 0, 1, lat));
 
     /* optional field ramp (hysteresis): steps of `sweeps_per` sweeps each */
     const snc_node *ramp = snc_lookup(&cf, "EfieldRamp");
@@ -279,6 +359,11 @@ int main(int argc, char **argv)
                 snprintf(base, sizeof(base), "T_%04d_%d_%03d-RDF", (int)Ts[i % nT], (int)r.J, m);
                 rdf_name(name, sizeof(name), base, ".dat", i, R);
                 CHECK(ctx, append_rdf(ctx, i, name, (int)Ts[i % nT]));
+            }
+            if (want_pot || want_cube || want_efield) {   /* src/starrynight-main.c:85-92 */
+                char prefix[128];
+                snprintf(prefix, sizeof(prefix), "T_%04d_%d_%03d", (int)Ts[i % nT], (int)r.J, m);
+                CHECK(ctx, write_maps(ctx, i, R, prefix, r.X, r.Y, r.Z, want_pot, want_cube, want_efield, lat));
             }
         }
         fflush(stdout);

@@ -53,6 +53,10 @@ struct sn_ctx {
     int max_tile_padded = 0;   // largest padded tile (field kernel's tile-order arrays)
     PairDev *d_pairs = nullptr, *d_offs = nullptr;
     RadialOff *d_radial = nullptr;   // radial-order offsets sorted by r^2 (built on first use)
+    StencilOff *d_stencil = nullptr;  // potential / field offsets (rebuilt when the cutoff changes)
+    int stencil_kind = -1, stencil_n = 0;  // -1 none, 0 potential, c > 0 field with cutoff c
+    double *d_obs = nullptr;          // observable output staging
+    size_t obs_cap = 0;
     int *d_radial_bins = nullptr;
     std::vector<int> radial_bins;
     unsigned *d_sweep = nullptr;
@@ -454,7 +458,8 @@ int sn_destroy(sn_ctx *c)
     void *bufs[] = {c->d_lat, c->d_reps, c->d_tiles, c->d_shapes, c->d_class_start, c->d_class_ri,
                     c->d_field_steps, c->d_tile_coord, c->d_chunk_col, c->d_gsteps, c->d_gchunk, c->d_pairs,
                     c->d_offs, c->d_sweep,
-                    c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins};
+                    c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins,
+                    c->d_stencil, c->d_obs};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -951,6 +956,74 @@ int sn_radial_order(sn_ctx *c, int32_t r, double fe[81], double afe[81], int64_t
         afe[b] = cnt ? sums[2 * b + 1] / (double)cnt : 0.0;
     }
     return SN_OK;
+}
+
+// Offsets of dipole_potential (kind 0: 0 < |r| <= 6, src/starrynight-analysis.c:72-80)
+// or dipole_electricfield (kind = cutoff: 0 < |r| <= cutoff, :403-413), in the
+// reference's dx, dy, dz loop order; the dz range runs when Z == 1.
+static int ensure_stencil(sn_ctx *c, int kind)
+{
+    if (c->d_stencil && c->stencil_kind == kind) return SN_OK;
+    const int R = kind == 0 ? 6 : kind;
+    std::vector<StencilOff> offs;
+    for (int dx = -R; dx <= R; dx++)
+        for (int dy = -R; dy <= R; dy++)
+            for (int dz = -R; dz <= R; dz++) {
+                const int d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 == 0 || d2 > R * R) continue;
+                const double d = std::sqrt((double)d2), w = 1.0 / (d * d * d);
+                if (kind == 0) offs.push_back(StencilOff{dx, dy, dz, 0, dx * w, dy * w, dz * w, w});
+                else offs.push_back(StencilOff{dx, dy, dz, 0, dx / d, dy / d, dz / d, w});
+            }
+    if (c->d_stencil) (void)hipFree(c->d_stencil);
+    c->d_stencil = nullptr;
+    HIPCHK(c, hipMalloc((void **)&c->d_stencil, offs.size() * sizeof(StencilOff)));
+    HIPCHK(c, hipMemcpy(c->d_stencil, offs.data(), offs.size() * sizeof(StencilOff), hipMemcpyHostToDevice));
+    c->stencil_kind = kind;
+    c->stencil_n = (int)offs.size();
+    return SN_OK;
+}
+
+static int run_stencil(sn_ctx *c, int32_t r, int kind, double *host_out, int width)
+{
+    const RepHost &h = c->reps[r];
+    int st;
+    if ((st = ensure_stencil(c, kind))) return st;
+    const size_t need = (size_t)h.N * width;
+    if (need > c->obs_cap) {
+        if (c->d_obs) (void)hipFree(c->d_obs);
+        c->d_obs = nullptr;
+        HIPCHK(c, hipMalloc((void **)&c->d_obs, need * sizeof(double)));
+        c->obs_cap = need;
+    }
+    const unsigned nb = (unsigned)((h.N + 255) / 256);
+    if (kind == 0)
+        hipLaunchKernelGGL(stencil_observable_kernel<false>, dim3(nb), dim3(256), 0, c->stream, c->d_lat, c->d_reps,
+                           r, c->d_stencil, c->stencil_n, c->d_obs);
+    else
+        hipLaunchKernelGGL(stencil_observable_kernel<true>, dim3(nb), dim3(256), 0, c->stream, c->d_lat, c->d_reps,
+                           r, c->d_stencil, c->stencil_n, c->d_obs);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(host_out, c->d_obs, need * sizeof(double), hipMemcpyDeviceToHost));
+    return SN_OK;
+}
+
+int sn_dipole_potential(sn_ctx *c, int32_t r, double *pot)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!pot) return fail(c, SN_ERR_INVALID, "null output");
+    return run_stencil(c, r, 0, pot, 1);
+}
+
+int sn_dipole_efield(sn_ctx *c, int32_t r, int32_t cutoff, double *E)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!E) return fail(c, SN_ERR_INVALID, "null output");
+    if (cutoff < 1 || cutoff > 9) return fail(c, SN_ERR_RANGE, "field cutoff must be in 1..9");
+    return run_stencil(c, r, cutoff, E, 3);
 }
 
 int sn_schedule_info(sn_ctx *c, int32_t r, int64_t *n_sites, int32_t *n_phases)

@@ -331,4 +331,53 @@ __global__ __launch_bounds__(256) void radial_order_kernel(const double4 *lat, c
         partials[(long long)blockIdx.x * 2 * kRadialBins + i] = (hist[0][i] + hist[1][i]) + (hist[2][i] + hist[3][i]);
 }
 
+// dipole_potential (src/starrynight-analysis.c:65-94) and dipole_electricfield
+// (src/starrynight-analysis.c:393-457) at every site of a replica, FP64.  One
+// lane per site; the offset loop is uniform across the wave, so the table is
+// read through scalar loads.  Potential offsets carry a = r / |r|^3,
+// field offsets n = r / |r| and w = 1 / |r|^3.
+struct StencilOff {
+    int dx, dy, dz, pad;
+    double ax, ay, az, w;
+};
+
+template <bool FIELD>
+__global__ __launch_bounds__(256) void stencil_observable_kernel(const double4 *lat, const RepDev *reps,
+                                                                 int replica, const StencilOff *offs, int noff,
+                                                                 double *out)
+{
+    const RepDev rep = reps[replica];
+    const int X = rep.X, Y = rep.Y, Z = rep.Z;
+    const long long N = (long long)X * Y * Z;
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    const int z = (int)(s % Z), y = (int)((s / Z) % Y), x = (int)(s / ((long long)Y * Z));
+    // x + dx + 9X >= 0 for |dx| <= 9 X: one unsigned modulo wraps any axis
+    const unsigned bx = 9u * X, by = 9u * Y, bz = 9u * Z;
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    for (int k = 0; k < noff; k++) {
+        const StencilOff o = offs[k];
+        const unsigned xx = (unsigned)(x + o.dx + (int)bx) % (unsigned)X;
+        const unsigned yy = (unsigned)(y + o.dy + (int)by) % (unsigned)Y;
+        const unsigned zz = (unsigned)(z + o.dz + (int)bz) % (unsigned)Z;
+        const double4 q = lat[rep.base + ((long long)xx * Y + yy) * Z + zz];
+        if (FIELD) {
+            const double radial = o.ax * q.x + o.ay * q.y + o.az * q.z;
+            ex += (3.0 * o.ax * radial - q.x) * o.w;
+            ey += (3.0 * o.ay * radial - q.y) * o.w;
+            ez += (3.0 * o.az * radial - q.z) * o.w;
+        } else {
+            ex += q.w * (q.x * o.ax + q.y * o.ay + q.z * o.az);
+        }
+    }
+    if (FIELD) {
+        const double4 a = lat[rep.base + s];
+        out[3 * s] = ex - a.x / 3.0;
+        out[3 * s + 1] = ey - a.y / 3.0;
+        out[3 * s + 2] = ez - a.z / 3.0;
+    } else {
+        out[s] = ex;
+    }
+}
+
 }  // namespace sn

@@ -222,6 +222,19 @@ class StarryNight:
                                         _ptr(cnt, C.c_int64)))
         return fe, afe, cnt
 
+    def dipole_potential(self, replica: int = 0) -> np.ndarray:
+        """dipole_potential (src/starrynight-analysis.c:65-94) at every site."""
+        pot = np.empty(self.replicas[replica].N, np.float64)
+        self._chk(lib().sn_dipole_potential(self._h, replica, _ptr(pot, C.c_double)))
+        return pot
+
+    def dipole_efield(self, cutoff: int, replica: int = 0) -> np.ndarray:
+        """dipole_electricfield (src/starrynight-analysis.c:393-457) at every
+        site, as (N, 3) vectors (the reference reports their norm)."""
+        E = np.empty((self.replicas[replica].N, 3), np.float64)
+        self._chk(lib().sn_dipole_efield(self._h, replica, int(cutoff), _ptr(E, C.c_double)))
+        return E
+
     # -- schedule / timing
     def schedule(self, replica: int = 0):
         n, ph = C.c_int64(0), C.c_int32(0)

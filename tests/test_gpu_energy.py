@@ -91,3 +91,22 @@ def test_radial_order_parity(sn, oracle, dims, mixed):
     few, afew, cntw = o.radial_order(lat)
     assert np.array_equal(cnt, cntw)
     assert np.abs(fe - few).max() <= 1e-12 and np.abs(afe - afew).max() <= 1e-12
+
+
+@pytest.mark.parametrize("dims,mixed", [((20, 20, 20), True), ((25, 25, 1), False), ((13, 11, 9), True)])
+def test_dipole_potential_and_field_parity(sn, oracle, dims, mixed):
+    """sn_dipole_potential / sn_dipole_efield vs the oracle's restatements of
+    dipole_potential and dipole_electricfield (cutoff 4, as the reference's
+    lattice_Efield_XYZ uses): FP64, only the summation order differs."""
+    rp = sn.ReplicaParams(*dims)
+    print_case("potential_efield", dims=dims, mixed=mixed)
+    lat = rand_lattice(rp.N, np.random.default_rng(sum(dims) + 1), mixed=mixed)
+    with sn.StarryNight(rp) as ctx:
+        ctx.set_lattice(lat)
+        pot = ctx.dipole_potential()
+        E4 = ctx.dipole_efield(4)
+        E2 = ctx.dipole_efield(2)
+    o = oracle_model(oracle, rp)
+    assert np.abs(pot - o.dipole_potential(lat)).max() <= 1e-12
+    assert np.abs(E4 - o.dipole_efield(lat, 4)).max() <= 1e-12
+    assert np.abs(E2 - o.dipole_efield(lat, 2)).max() <= 1e-12

@@ -56,6 +56,9 @@ MCEqmSteps: 1
 MCMegaSteps: 3
 MCMoves: 4.0
 CalculateRadialOrderParameter: true
+CalculatePotential: true
+SavePotentialCube: true
+CalculateEfield: true
 SaveLattice = "final.bin";
 """
 
@@ -91,6 +94,19 @@ def test_host_reference_run(sn, oracle, tmp_path):
     e_last = float(rows[-1][3])
     o = oracle.Oracle(24, 24, 24, 3, J=2.0, E=(0.0, 0.0, float(np.float32(0.02))))
     assert abs(o.energy(fin)[0] / 24 ** 3 - e_last) < 2e-6
+    # the last megastep's potential / field maps describe the final lattice
+    pot = np.loadtxt(tmp_path / "T_0250_2_002_potential.xyz")
+    assert pot.shape == (24 ** 3, 4) and np.array_equal(pot[:3, :3], [[0, 0, 0], [0, 0, 1], [0, 0, 2]])
+    assert np.abs(pot[:, 3] - o.dipole_potential(fin)).max() <= 6e-7
+    ef = np.loadtxt(tmp_path / "T_0250_2_002_efield.xyz")
+    assert np.abs(ef[:, 3] - np.linalg.norm(o.dipole_efield(fin, 4), axis=1)).max() <= 6e-7
+    assert (tmp_path / "equilib_lattice_efield.xyz").exists()
+    cube = (tmp_path / "T_0250_2_002_potential.cube").read_text().split("\n")
+    assert cube[0] == "Starrynight Cube file: 24 24 24" and cube[2] == "1 0.0 0.0 0.0"
+    assert cube[3:7] == ["24 1.0 0.0 0.0", "24 0.0 1.0 0.0", "24 0.0 0.0 1.0", "1 0.0 0.0 0.0 0.0 0.0"]
+    vals = np.array([float(v) for line in cube[7:] for v in line.split()])
+    assert vals.shape == (24 ** 3,)
+    assert np.allclose(vals, pot[:, 3], atol=1e-5, rtol=1e-5)
 
 
 def test_host_batched_schedule(sn, tmp_path):

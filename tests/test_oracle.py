@@ -382,3 +382,59 @@ def test_library_exports_every_declared_symbol(sn):
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing
     assert declared == set(sn.EXPORTS)
+
+
+@pytest.mark.parametrize("dims", [(14, 14, 14), (15, 13, 1)])
+def test_dipole_potential_and_field_single_dipole(oracle, dims):
+    """One dipole p at s (length L), every other site empty: the potential at
+    i is the textbook L p.(s-i)/|s-i|^3 within |s-i| <= 6, the field at i is
+    (3 n (n.p) - p)/d^3 with n = (s-i)/d within the cutoff (lengths not
+    applied), and -p/3 at s itself; zero beyond the cutoffs."""
+    X, Y, Z = dims
+    N = X * Y * Z
+    lat = np.zeros((N, 4))
+    s = np.array([5, 6, 0 if Z == 1 else 7])
+    p = np.array([0.3, -0.5, 0.81]); p /= np.linalg.norm(p)
+    k = (s[0] * Y + s[1]) * Z + s[2]
+    lat[k, :3] = p; lat[k, 3] = 0.7
+    o = oracle.Oracle(X, Y, Z, 3)
+    pot = o.dipole_potential(lat)
+    E = o.dipole_efield(lat, 4)
+    idx = np.arange(N)
+    xyz = np.stack([idx // (Y * Z), (idx // Z) % Y, idx % Z], 1)
+    d = s[None, :] - xyz
+    d = (d + np.array([X, Y, Z]) // 2) % np.array([X, Y, Z]) - np.array([X, Y, Z]) // 2   # minimum image
+    if Z == 1:
+        # the dz range runs over -6..6 onto the same plane: each in-plane
+        # offset counts once per dz with |r| <= cutoff
+        want_pot = np.zeros(N); want_E = np.zeros((N, 3))
+        for dz in range(-6, 7):
+            r = np.concatenate([d[:, :2], np.full((N, 1), dz)], 1).astype(float)
+            r2 = (r * r).sum(1)
+            ok = (r2 > 0) & (r2 <= 36)
+            want_pot[ok] += 0.7 * (r[ok] @ p) / r2[ok] ** 1.5
+            ok4 = (r2 > 0) & (r2 <= 16)
+            n = r[ok4] / np.sqrt(r2[ok4])[:, None]
+            want_E[ok4] += (3 * n * (n @ p)[:, None] - p) / r2[ok4, None] ** 1.5
+    else:
+        r = d.astype(float)
+        r2 = (r * r).sum(1)
+        ok = (r2 > 0) & (r2 <= 36)
+        want_pot = np.where(ok, 0.7 * (r @ p) / np.maximum(r2, 1) ** 1.5, 0.0)
+        ok4 = (r2 > 0) & (r2 <= 16)
+        n = r / np.sqrt(np.maximum(r2, 1))[:, None]
+        want_E = np.where(ok4[:, None], (3 * n * (n @ p)[:, None] - p) / np.maximum(r2, 1)[:, None] ** 1.5, 0.0)
+    want_E[k] -= p / 3
+    assert np.abs(pot - want_pot).max() < 1e-13
+    assert np.abs(E - want_E).max() < 1e-13
+
+
+def test_dipole_potential_and_field_ferroelectric(oracle):
+    """Uniform lattice: the potential sums p.r/|r|^3 over a symmetric shell (0),
+    the field sums to zero by cubic symmetry, leaving -p/3 (|E| = 1/3)."""
+    X = 12
+    lat = np.zeros((X ** 3, 4)); lat[:, 0] = 1.0; lat[:, 3] = 1.0
+    o = oracle.Oracle(X, X, X, 3)
+    assert np.abs(o.dipole_potential(lat)).max() < 1e-13
+    E = o.dipole_efield(lat, 3)
+    assert np.abs(E - np.array([-1 / 3, 0, 0])).max() < 1e-13
