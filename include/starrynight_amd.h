@@ -191,6 +191,11 @@ int sn_dipole_potential(sn_ctx *ctx, int32_t replica, double *pot);
  * (lengths not applied, as in the reference, which reports |E[i]|).
  * E: double[N][3]; cutoff in 1..9. */
 int sn_dipole_efield(sn_ctx *ctx, int32_t replica, int32_t cutoff, double *E);
+/* recombination_calculator (src/starrynight-analysis.c:97-164) of replica r
+ * from its dipole potential (0.165 / 5 eV per unit, kT = 0.025 eV), FP64:
+ * out = {ZBe, ZBh, ZFDe, ZFDh, R_Boltz = N^2 / (ZBe ZBh), R_FD = N sum e_i h_i,
+ * FD-Total-electron, FD-Total-hole} -- the numbers of its log line. */
+int sn_recombination(sn_ctx *ctx, int32_t replica, double out[8]);
 
 /* ---- schedule introspection (pure host logic, no device work) ----------- */
 

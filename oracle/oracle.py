@@ -97,6 +97,7 @@ def lib():
         L.or_radial_order_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, P(C.c_int64)]
         L.or_dipole_potential_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
         L.or_dipole_efield_f64.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.or_recombination_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -240,6 +241,11 @@ class Oracle:
         E = np.empty((self.N, 3), np.float64)
         lib().or_dipole_efield_f64(self.X, self.Y, self.Z, int(cutoff), _p(lat64, C.c_double), _p(E, C.c_double))
         return E
+
+    def recombination(self, lat64):
+        out = np.empty(8, np.float64)
+        lib().or_recombination_f64(self.X, self.Y, self.Z, _p(lat64, C.c_double), _p(out, C.c_double))
+        return out
 
     def radial_order(self, lat64):
         fe = np.empty(81); afe = np.empty(81); cnt = np.empty(81, np.int64)

@@ -610,3 +610,34 @@ void or_dipole_efield_f64(int X, int Y, int Z, int cutoff, const double *lat, do
                 for (int k = 0; k < 3; k++) o[k] = e[k] - a[k] / 3.0;
             }
 }
+
+/* recombination_calculator (src/starrynight-analysis.c:97-164), FP64:
+ * pot_i = 0.165 / 5 * dipole_potential_i (eV), beta = 1 / 0.025 eV^-1,
+ * Boltzmann and Fermi-Dirac partition sums, FD densities normalised by their
+ * partition sums, R_Boltz = N^2 / (ZBe ZBh), R_FD = N sum_i e_i h_i. */
+void or_recombination_f64(int X, int Y, int Z, const double *lat, double out[8])
+{
+    const int64_t N = (int64_t)X * Y * Z;
+    double *pot = (double *)malloc(sizeof(double) * N);
+    or_dipole_potential_f64(X, Y, Z, lat, pot);
+    const double BETA = 1 / (0.025), scale = 0.165 / 5;
+    double ZBe = 0, ZBh = 0, ZFDe = 0, ZFDh = 0;
+    for (int64_t i = 0; i < N; i++) {
+        const double p = scale * pot[i];
+        ZBe += exp(-p * BETA);
+        ZBh += exp(p * BETA);
+        ZFDe += 1.0 / (exp(p * BETA) + 1.0);
+        ZFDh += 1.0 / (exp(-p * BETA) + 1.0);
+    }
+    double et = 0, ht = 0, rt = 0;
+    for (int64_t i = 0; i < N; i++) {
+        const double p = scale * pot[i];
+        const double e = 1.0 / (exp(p * BETA) + 1.0) / ZFDe, h = 1.0 / (exp(-p * BETA) + 1.0) / ZFDh;
+        et += e; ht += h; rt += e * h;
+    }
+    free(pot);
+    out[0] = ZBe; out[1] = ZBh; out[2] = ZFDe; out[3] = ZFDh;
+    out[4] = (double)N * (double)N / (ZBe * ZBh);
+    out[5] = (double)N * rt;
+    out[6] = et; out[7] = ht;
+}

@@ -131,6 +131,10 @@ void or_dipole_potential_f64(int X, int Y, int Z, const double *lat, double *pot
  * - p_i / 3 (lengths are not applied, as in the reference); the reference
  * returns |E_i|. */
 void or_dipole_efield_f64(int X, int Y, int Z, int cutoff, const double *lat, double *E);
+/* recombination_calculator (src/starrynight-analysis.c:97-164), FP64, from
+ * the dipole potential: out = {ZBe, ZBh, ZFDe, ZFDh, R_Boltz, R_FD,
+ * FD-Total-electron, FD-Total-hole}. */
+void or_recombination_f64(int X, int Y, int Z, const double *lat, double out[8]);
 
 #ifdef __cplusplus
 }

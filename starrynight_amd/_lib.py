@@ -23,7 +23,7 @@ EXPORTS = [
     "sn_get_lattice", "sn_lattice_device_ptr", "sn_init_lattice_random", "sn_set_params",
     "sn_site_energy_batch", "sn_site_energy_batch_device", "sn_mc_sweeps", "sn_sync",
     "sn_mc_replay", "sn_counters", "sn_reset_counters", "sn_energy", "sn_gradient", "sn_moment",
-    "sn_radial_order", "sn_dipole_potential", "sn_dipole_efield",
+    "sn_radial_order", "sn_dipole_potential", "sn_dipole_efield", "sn_recombination",
     "sn_schedule_info", "sn_schedule_order", "sn_stream", "sn_sweep_config",
     "sn_time_sweep_kernel", "sn_plan_schedule", "sn_autotune", "sn_reference_initial_lattice",
 ]
@@ -82,6 +82,7 @@ def lib():
         L.sn_radial_order.argtypes = [vp, C.c_int32, dp, dp, C.POINTER(C.c_int64)]
         L.sn_dipole_potential.argtypes = [vp, C.c_int32, dp]
         L.sn_dipole_efield.argtypes = [vp, C.c_int32, C.c_int32, dp]
+        L.sn_recombination.argtypes = [vp, C.c_int32, dp]
         L.sn_schedule_info.argtypes = [vp, C.c_int32, P(C.c_int64), ip]
         L.sn_schedule_order.argtypes = [vp, C.c_int32, ip, ip]
         L.sn_stream.argtypes = [vp, P(vp)]

@@ -269,8 +269,9 @@ int main(int argc, char **argv)
                 tun.tile[0], tun.tile[1], tun.tile[2], tun.fine_colouring, tun.kernel, tun.block_threads,
                 1e-6 * rate);
     }
-    int rdf = 0, want_pot = 0, want_cube = 0, want_efield = 0;
+    int rdf = 0, want_pot = 0, want_cube = 0, want_efield = 0, want_recomb = 0;
     snc_lookup_bool(&cf, "CalculateRadialOrderParameter", &rdf);
+    snc_lookup_bool(&cf, "CalculateRecombination", &want_recomb);
     snc_lookup_bool(&cf, "CalculatePotential", &want_pot);
     snc_lookup_bool(&cf, "SavePotentialCube", &want_cube);
     snc_lookup_bool(&cf, "CalculateEfield", &want_efield);
@@ -291,6 +292,18 @@ int main(int argc, char **argv)
         uint32_t mt_seed = (uint32_t)(0xDEADBEEFu + (uint32_t)(int)Ts[i % nT]);
         CHECK(ctx, sn_reference_initial_lattice(r.X, r.Y, r.Z, r.initial, mt_seed, r.nspec, r.len, r.prev, lat));
         CHECK(ctx, sn_set_lattice(ctx, i, lat));
+    }
+    /* the run log "Recombination_T_%04d.log" and its header (src/starrynight-main.c:164-178) */
+    FILE **logs = (FILE **)calloc(R, sizeof(FILE *));
+    if (!logs) return 1;
+    for (int i = 0; i < R; i++) {
+        char base[64], name[256];
+        snprintf(base, sizeof(base), "Recombination_T_%04d", (int)Ts[i % nT]);
+        rdf_name(name, sizeof(name), base, ".log", i, R);
+        logs[i] = fopen(name, "w");
+        if (!logs[i]) { fprintf(stderr, "cannot open %s\n", name); return 1; }
+        fprintf(logs[i], "# Starrynight - simulation run on time(NULL)= %ld\n# Mersenne Twister Seed: %X\n",
+                (long)time(NULL), (unsigned)(0xDEADBEEFu + (uint32_t)(int)Ts[i % nT]));
     }
     if (rdf)   /* analysis_initial (src/starrynight-main.c:39) */
         for (int i = 0; i < R; i++) {
@@ -360,6 +373,14 @@ int main(int argc, char **argv)
                 rdf_name(name, sizeof(name), base, ".dat", i, R);
                 CHECK(ctx, append_rdf(ctx, i, name, (int)Ts[i % nT]));
             }
+            if (want_recomb) {   /* recombination_calculator's log line (src/starrynight-analysis.c:127-162) */
+                double rc_[8];
+                CHECK(ctx, sn_recombination(ctx, i, rc_));
+                fprintf(logs[i], "T: %d ZBe: %e ZBh: %e ZFDe: %e ZFDh: %e R_Boltz: %e ", (int)Ts[i % nT], rc_[0],
+                        rc_[1], rc_[2], rc_[3], rc_[4]);
+                fprintf(logs[i], "R_FD: %e FD-Total-electron: %e FD-Total-hole: %e\n", rc_[5], rc_[6], rc_[7]);
+                fflush(logs[i]);
+            }
             if (want_pot || want_cube || want_efield) {   /* src/starrynight-main.c:85-92 */
                 char prefix[128];
                 snprintf(prefix, sizeof(prefix), "T_%04d_%d_%03d", (int)Ts[i % nT], (int)r.J, m);
@@ -384,6 +405,8 @@ int main(int argc, char **argv)
         if (!f || fwrite(lat, sizeof(double) * 4, N, f) != (size_t)N) { fprintf(stderr, "cannot write %s\n", save); return 1; }
         fclose(f);
     }
+    for (int i = 0; i < R; i++) fclose(logs[i]);
+    free(logs);
     free(lat);
     free(ps);
     sn_destroy(ctx);

@@ -1026,6 +1026,35 @@ int sn_dipole_efield(sn_ctx *c, int32_t r, int32_t cutoff, double *E)
     return run_stencil(c, r, cutoff, E, 3);
 }
 
+int sn_recombination(sn_ctx *c, int32_t r, double out[8])
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!out) return fail(c, SN_ERR_INVALID, "null output");
+    const RepHost &h = c->reps[r];
+    std::vector<double> pot(h.N);
+    if ((st = run_stencil(c, r, 0, pot.data(), 1))) return st;   // leaves the potential in d_obs
+    const int nblk = (int)std::min<long long>((h.N + 255) / 256, 1024);
+    if ((st = ensure_partials(c, (size_t)nblk * kRecombParts))) return st;
+    // the reference's constants: 0.165 eV per internal unit, screened by 5; kT = 0.025 eV
+    hipLaunchKernelGGL(recombination_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_obs, (long long)h.N,
+                       0.165 / 5, 1 / (0.025), c->d_partials);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk,
+                       kRecombParts, c->d_result);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double v[kRecombParts];
+    HIPCHK(c, hipMemcpy(v, c->d_result, sizeof(v), hipMemcpyDeviceToHost));
+    const double N = (double)h.N;
+    out[0] = v[0]; out[1] = v[1]; out[2] = v[2]; out[3] = v[3];
+    out[4] = N * N / (v[0] * v[1]);
+    out[5] = N * v[4] / (v[2] * v[3]);
+    out[6] = v[2] / v[2];   // sum of the normalised electron density (1 up to rounding)
+    out[7] = v[3] / v[3];
+    return SN_OK;
+}
+
 int sn_schedule_info(sn_ctx *c, int32_t r, int64_t *n_sites, int32_t *n_phases)
 {
     int st = check_rep(c, r);

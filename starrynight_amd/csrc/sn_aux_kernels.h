@@ -380,4 +380,31 @@ __global__ __launch_bounds__(256) void stencil_observable_kernel(const double4 *
     }
 }
 
+// recombination_calculator's sums (src/starrynight-analysis.c:97-164) over a
+// potential map: per workgroup partial sums of {exp(-b p), exp(b p),
+// 1 / (exp(b p) + 1), 1 / (exp(-b p) + 1), their product}, p = scale * pot.
+constexpr int kRecombParts = 5;
+
+__global__ __launch_bounds__(256) void recombination_kernel(const double *pot, long long N, double scale,
+                                                            double beta, double *partials)
+{
+    __shared__ double red[kRecombParts][256];
+    const int tid = threadIdx.x;
+    double v[kRecombParts] = {0, 0, 0, 0, 0};
+    for (long long i = (long long)blockIdx.x * 256 + tid; i < N; i += (long long)gridDim.x * 256) {
+        const double p = scale * pot[i];
+        const double ep = exp(p * beta), em = exp(-p * beta);
+        const double fe = 1.0 / (ep + 1.0), fh = 1.0 / (em + 1.0);
+        v[0] += em; v[1] += ep; v[2] += fe; v[3] += fh; v[4] += fe * fh;
+    }
+    for (int c = 0; c < kRecombParts; c++) red[c][tid] = v[c];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st)
+            for (int c = 0; c < kRecombParts; c++) red[c][tid] += red[c][tid + st];
+        __syncthreads();
+    }
+    if (tid < kRecombParts) partials[(long long)blockIdx.x * kRecombParts + tid] = red[tid][0];
+}
+
 }  // namespace sn

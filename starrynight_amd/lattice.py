@@ -235,6 +235,13 @@ class StarryNight:
         self._chk(lib().sn_dipole_efield(self._h, replica, int(cutoff), _ptr(E, C.c_double)))
         return E
 
+    def recombination(self, replica: int = 0) -> np.ndarray:
+        """recombination_calculator (src/starrynight-analysis.c:97-164):
+        {ZBe, ZBh, ZFDe, ZFDh, R_Boltz, R_FD, FD-Total-electron, FD-Total-hole}."""
+        out = np.empty(8, np.float64)
+        self._chk(lib().sn_recombination(self._h, replica, _ptr(out, C.c_double)))
+        return out
+
     # -- schedule / timing
     def schedule(self, replica: int = 0):
         n, ph = C.c_int64(0), C.c_int32(0)

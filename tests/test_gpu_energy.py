@@ -110,3 +110,15 @@ def test_dipole_potential_and_field_parity(sn, oracle, dims, mixed):
     assert np.abs(pot - o.dipole_potential(lat)).max() <= 1e-12
     assert np.abs(E4 - o.dipole_efield(lat, 4)).max() <= 1e-12
     assert np.abs(E2 - o.dipole_efield(lat, 2)).max() <= 1e-12
+
+
+@pytest.mark.parametrize("dims", [(20, 20, 20), (25, 25, 1)])
+def test_recombination_parity(sn, oracle, dims):
+    rp = sn.ReplicaParams(*dims)
+    print_case("recombination", dims=dims)
+    lat = rand_lattice(rp.N, np.random.default_rng(5), mixed=True)
+    with sn.StarryNight(rp) as ctx:
+        ctx.set_lattice(lat)
+        got = ctx.recombination()
+    want = oracle_model(oracle, rp).recombination(lat)
+    assert np.allclose(got, want, rtol=1e-11, atol=0)

@@ -59,6 +59,7 @@ CalculateRadialOrderParameter: true
 CalculatePotential: true
 SavePotentialCube: true
 CalculateEfield: true
+CalculateRecombination: true
 SaveLattice = "final.bin";
 """
 
@@ -107,6 +108,15 @@ def test_host_reference_run(sn, oracle, tmp_path):
     vals = np.array([float(v) for line in cube[7:] for v in line.split()])
     assert vals.shape == (24 ** 3,)
     assert np.allclose(vals, pot[:, 3], atol=1e-5, rtol=1e-5)
+    # recombination log: header, then one line per megastep; the last describes the final lattice
+    log = (tmp_path / "Recombination_T_0250.log").read_text().splitlines()
+    assert log[0].startswith("# Starrynight") and log[1] == "# Mersenne Twister Seed: %X" % (0xDEADBEEF + 250)
+    assert len(log) == 2 + 3
+    f = log[-1].split()
+    assert f[0] == "T:" and f[1] == "250" and f[12] == "R_FD:"
+    want = o.recombination(fin)
+    got = np.array([float(f[i]) for i in (3, 5, 7, 9, 11, 13, 15, 17)])
+    assert np.allclose(got, want, rtol=2e-6)
 
 
 def test_host_batched_schedule(sn, tmp_path):

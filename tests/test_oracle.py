@@ -438,3 +438,23 @@ def test_dipole_potential_and_field_ferroelectric(oracle):
     assert np.abs(o.dipole_potential(lat)).max() < 1e-13
     E = o.dipole_efield(lat, 3)
     assert np.abs(E - np.array([-1 / 3, 0, 0])).max() < 1e-13
+
+
+def test_recombination_closed_forms(oracle):
+    """Uniform lattice: the potential vanishes, so every Boltzmann factor is 1
+    and every Fermi-Dirac occupation 1/2: ZB = N, ZFD = N/2, R_Boltz = 1,
+    R_FD = 1 and both FD totals 1.  A single dipole: the sums follow from the
+    (separately pinned) potential with numpy."""
+    X = 10
+    N = X ** 3
+    lat = np.zeros((N, 4)); lat[:, 2] = 1.0; lat[:, 3] = 1.0
+    o = oracle.Oracle(X, X, X, 3)
+    r = o.recombination(lat)
+    assert np.allclose(r, [N, N, N / 2, N / 2, 1.0, 1.0, 1.0, 1.0], rtol=1e-13, atol=0)
+    lat = np.zeros((N, 4)); lat[123, :3] = (0.0, 0.6, 0.8); lat[123, 3] = 1.0
+    pot = 0.165 / 5 * o.dipole_potential(lat)
+    b = 40.0
+    zbe, zbh = np.exp(-pot * b).sum(), np.exp(pot * b).sum()
+    fe, fh = 1 / (np.exp(pot * b) + 1), 1 / (np.exp(-pot * b) + 1)
+    want = [zbe, zbh, fe.sum(), fh.sum(), N * N / (zbe * zbh), N * ((fe / fe.sum()) * (fh / fh.sum())).sum(), 1, 1]
+    assert np.allclose(o.recombination(lat), want, rtol=1e-12, atol=0)
