@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the headline sweeps (for profiler runs)")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the other configurations (2D, temperature sweep, field ramp, adjoint)")
     ap.add_argument("--copy-kernel", action="store_true",
                     help="run only a 64 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
@@ -209,6 +211,10 @@ def main():
     if not args.no_batched:
         batched = run_batched(sn, rc, log, autotune=not args.no_autotune)
 
+    workloads = None
+    if not args.no_workloads:
+        workloads = other_workloads(sn, log, autotune=not args.no_autotune)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
@@ -231,9 +237,80 @@ def main():
         "speedup_vs_cpu_all_cores": (rate / cpu["value"]) if cpu else None,
         "speedup_vs_cpu_one_core": (rate / cpu["single_core"]) if cpu else None,
         "batched": batched,
+        "workloads": workloads,
         "pcie_inclusive": pcie,
     }
     print(json.dumps(line))
+
+
+def _timed_sweeps(ctx, sweeps, sites):
+    ctx.MC_sweeps(2)
+    t0 = time.perf_counter()
+    ctx.MC_sweeps(sweeps, sync=False)
+    ctx.sync()
+    return sweeps * sites / (time.perf_counter() - t0)
+
+
+def other_workloads(sn, log, autotune=True):
+    """The remaining configurations, each timed briefly (site-updates/s of the
+    sweeps; the adjoint in gradient evaluations/s)."""
+    out = {}
+
+    def tuned(reps, ms):
+        return sn.autotune(reps, budget_ms=ms, use_cache=False)[0] if autotune else None
+
+    # 2D 25x25x1, rc = 3, T = 300 (the reference's CPU-runnable case): 1000 sweeps
+    rp = sn.ReplicaParams(25, 25, 1, DipoleCutOff=3, T=300, seed=0xDEADBEEF + 300)
+    with sn.StarryNight(rp, tuned(rp, 1500)) as ctx:
+        ctx.init_lattice_random(0)
+        out["2d_25x25_rc3"] = {"site_updates_per_s": _timed_sweeps(ctx, 1000, rp.N), "sweeps": 1000}
+    # 64^3 rc = 3: one replica, and the 0-500 K sweep in 25 K steps batched as 21 replicas
+    rp = sn.ReplicaParams(64, 64, 64, DipoleCutOff=3, T=300, seed=64)
+    with sn.StarryNight(rp, tuned(rp, 2000)) as ctx:
+        ctx.init_lattice_random(0)
+        single = _timed_sweeps(ctx, 20, rp.N)
+    reps = [sn.ReplicaParams(64, 64, 64, DipoleCutOff=3, T=25.0 * k, seed=640 + k) for k in range(21)]
+    with sn.StarryNight(reps, tuned(reps, 3000)) as ctx:
+        for r in range(len(reps)):
+            ctx.init_lattice_random(r)
+        agg = _timed_sweeps(ctx, 10, sum(r.N for r in reps))
+    out["64cube_rc3_temperature_sweep"] = {"single_replica": single, "21_temperatures_batched": agg,
+                                           "temperatures": "0..500 K step 25"}
+    # 128^3 rc = 4 solid solution (Dipoles [1, 0], Prevalence [0.5, 0.5]), 16 replicas
+    # on an E_z ramp: 3 ramp steps of 3 sweeps each (the full ramp is 41 x 500)
+    reps = [sn.ReplicaParams(128, 128, 128, DipoleCutOff=4, T=300, Efield=(0, 0, -0.1), seed=4000 + k)
+            for k in range(16)]
+    with sn.StarryNight(reps, tuned(reps, 4000)) as ctx:
+        for r in range(len(reps)):
+            ctx.init_lattice_random(r, Dipoles=(1.0, 0.0), Prevalence=(0.5, 0.5))
+        ctx.MC_sweeps(1)
+        t0 = time.perf_counter()
+        steps, per = 3, 3
+        for k in range(steps):
+            for r in range(len(reps)):
+                ctx.set_params(r, 300, (0.0, 0.0, -0.1 + 0.005 * k), 1.0)
+            ctx.MC_sweeps(per, sync=False)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        n = sum(r.N for r in reps)
+        out["128cube_rc4_solid_solution_ramp"] = {"site_updates_per_s": steps * per * n / dt,
+                                                  "replicas": len(reps), "sweeps": steps * per}
+    # 32 replicas of 48^3, rc = 3: energy + adjoint (local field and parameter derivatives)
+    reps = [sn.ReplicaParams(48, 48, 48, DipoleCutOff=3, T=300, seed=480 + k) for k in range(32)]
+    with sn.StarryNight(reps) as ctx:
+        for r in range(len(reps)):
+            ctx.init_lattice_random(r)
+        ctx.gradient(0)
+        t0 = time.perf_counter()
+        for r in range(len(reps)):
+            ctx.gradient(r)
+        dt = time.perf_counter() - t0
+        out["32x48cube_adjoint"] = {"gradients_per_s": len(reps) / dt, "replicas": len(reps),
+                                    "note": "each gradient: dE/dp on 110592 sites + dE/d(J, E, s, K), "
+                                            "including the host copy of the per-site field"}
+    for k, v in out.items():
+        log(f"[bench] {k}: {v}")
+    return out
 
 
 def batched_replicas(sn, rc):
