@@ -36,6 +36,7 @@ struct sn_ctx {
     sn_tuning tuning{};
     int G = 16, NPL = 4, npairs = 0, noff = 0;
     int blk = 256;   // gather kernel's threads per workgroup
+    int prop_chunk = kBlock;   // gather kernel's proposals per chunk (LDS slots)
     long long total_sites = 0;
     size_t lds_bytes = 0;
     hipStream_t stream = nullptr;
@@ -162,20 +163,21 @@ constexpr int kMaxRegion = 4096;
 // (3D) or 1 site (2D).
 constexpr int kMaxTile3D = kFieldMaxTile3D, kMaxTile2D = kFieldMaxTile2D;
 
-static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &t, Schedule &s)
+static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &t, Schedule &s,
+                 int chunk = kBlock)
 {
     int target[3] = {t.tile[0] > 0 ? t.tile[0] : 8, t.tile[1] > 0 ? t.tile[1] : 8,
                      t.tile[2] > 0 ? t.tile[2] : 8};
     bool two_d = geom[0].Z == 1;
     int kind = t.fine_colouring;
     if (kind == 0) kind = (!two_d && rc == 3) ? 3 : 2;
-    return build_schedule(geom, rc, target, kind, kBlock, kMaxRegion, two_d ? kMaxTile2D : kMaxTile3D, s);
+    return build_schedule(geom, rc, target, kind, chunk, kMaxRegion, two_d ? kMaxTile2D : kMaxTile3D, s);
 }
 
 // dynamic LDS bytes of the per-phase gather kernel
 static size_t sweep_lds(const sn_ctx *c, bool mixed)
 {
-    size_t dbl = (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * kBlock) * sizeof(double);
+    size_t dbl = (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * c->prop_chunk) * sizeof(double);
     size_t ints = (size_t)(c->max_tile_sites + c->max_cls + 1 + 2 * (c->max_chunks + 1) +
                            c->max_gsteps * (c->blk / c->G + 1)) * sizeof(int);
     return dbl + ints;
@@ -275,6 +277,17 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     // A lattice that cannot be tiled still supports the gather entry points
     // (site_energy, replay, energy, gradient); only sweeps are refused.
     c->can_sweep = plan(c->geom, c->rc, c->tuning, c->sched);
+    // Tiles of at most 256 sites draw proposals in chunks of 128: the gather
+    // kernel's proposal buffer halves, and three workgroups fit in a CU's LDS.
+    if (c->can_sweep) {
+        int mt = 0;
+        for (const TileShape &sh_ : c->sched.shapes) mt = std::max(mt, sh_.tx * sh_.ty * sh_.tz);
+        Schedule small;
+        if (mt <= 256 && plan(c->geom, c->rc, c->tuning, small, 128)) {
+            c->sched = small;
+            c->prop_chunk = 128;
+        }
+    }
     if (!c->can_sweep) {
         c->sweep_err = c->sched.error;
         c->sched.shapes.clear();
@@ -620,6 +633,7 @@ static SweepArgs sweep_args(sn_ctx *c)
     a.class_start = c->d_class_start; a.class_ri = c->d_class_ri; a.chunk_col = c->d_chunk_col;
     a.gsteps = c->d_gsteps; a.gchunk = c->d_gchunk;
     a.max_gsteps = c->max_gsteps; a.max_chunks = c->max_chunks;
+    a.prop_stride = c->prop_chunk;
     a.pairs = c->d_pairs; a.sweep_ctr = c->d_sweep; a.counters = c->d_counters;
     a.npairs = c->npairs; a.rc = c->rc; a.two_d = c->two_d ? 1 : 0;
     a.max_region = c->sched.max_region;

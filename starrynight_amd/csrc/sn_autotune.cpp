@@ -149,7 +149,7 @@ extern "C" int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t 
     if (two_d)
         tiles = {{8, 8, 1}, {12, 12, 1}, {16, 16, 1}, {16, 8, 1}, {24, 24, 1}};
     else
-        tiles = {{8, 8, 8}, {8, 8, 6}, {8, 8, 10}, {6, 8, 8}, {6, 6, 6}, {8, 6, 6}, {4, 8, 8}};
+        tiles = {{8, 8, 8}, {8, 8, 6}, {8, 8, 10}, {6, 8, 8}, {6, 6, 6}, {8, 6, 6}, {4, 8, 8}, {8, 8, 4}, {4, 4, 8}};
     std::vector<int> colourings = (!two_d && rc == 3) ? std::vector<int>{3, 2, 1} : std::vector<int>{2, 1};
     std::vector<sn_tuning> cands;
     // (kernel, threads per tile): field-update; gather with 4 or 8 waves

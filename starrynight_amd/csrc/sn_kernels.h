@@ -30,6 +30,7 @@ struct SweepArgs {
     int npairs, rc, two_d, max_region;
     int max_tile_sites, max_cls;   // sizes of the LDS copies of the shape tables
     int max_gsteps, max_chunks;
+    int prop_stride;          // proposal slots per chunk (<= kBlock)
 };
 
 // DPP lane exchange of a double (two 32-bit moves); CTRL is a dpp_ctrl code.
@@ -76,7 +77,7 @@ __device__ __forceinline__ double group_sum_dpp(double v)
 __device__ __forceinline__ int s_gchunk_total(const ShapeDev &sh) { return sh.gsteps_n; }
 
 // BLK threads per workgroup (256 or 512); proposals are drawn in chunks of at
-// most kBlock sites.
+// most prop_stride (<= kBlock) sites.
 template <int G, int NPL, bool MIXED, int BLK>
 __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_tile, int ntiles)
 {
@@ -98,8 +99,9 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     double *sy = sx + a.max_region;
     double *sz = sy + a.max_region;
     double *sl = sz + a.max_region;                        // lengths (MIXED only)
-    double *prop = MIXED ? sl + a.max_region : sl;         // 4 x kBlock: px, py, pz, threshold
-    int *s_ri = (int *)(prop + 4 * kBlock);                // region index of class sites
+    double *prop = MIXED ? sl + a.max_region : sl;         // 4 x prop_stride: px, py, pz, threshold
+    const int PS = a.prop_stride;
+    int *s_ri = (int *)(prop + 4 * PS);                    // region index of class sites
     int *s_cls = s_ri + a.max_tile_sites;                  // class starts, then chunk colours
     int *s_chunk = s_cls + a.max_cls + 1;
     int *s_gchunk = s_chunk + a.max_chunks + 1;            // first step of each chunk
@@ -186,9 +188,9 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
             double px, py, pz, u;
             counter_draw(rep.seed, (uint32_t)tile.replica, sweep, gsite, rep.constrain_to_x, px, py, pz, u);
             prop[tid] = px;
-            prop[kBlock + tid] = py;
-            prop[2 * kBlock + tid] = pz;
-            prop[3 * kBlock + tid] = metropolis_threshold(u, rep.beta);
+            prop[PS + tid] = py;
+            prop[2 * PS + tid] = pz;
+            prop[3 * PS + tid] = metropolis_threshold(u, rep.beta);
         }
         __syncthreads();
         // this chunk's steps: every group decides one site per step; a step
@@ -226,8 +228,8 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                     hz_ += w[m] * mz + ts * nz[m];
                     if (m == 0) { gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz; }
                 }
-                const double px = prop[pq], py = prop[kBlock + pq], pz = prop[2 * kBlock + pq];
-                const double thr = prop[3 * kBlock + pq];
+                const double px = prop[pq], py = prop[PS + pq], pz = prop[2 * PS + pq];
+                const double thr = prop[3 * PS + pq];
                 const double ox = sx[ri], oy = sy[ri], oz = sz[ri];
                 const double Li = MIXED ? sl[ri] : rep.L0;
                 const double coef = MIXED ? Li : Li * rep.L0;
