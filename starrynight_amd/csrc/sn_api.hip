@@ -209,9 +209,14 @@ int sn_plan_schedule(const sn_params *params, int32_t n_replicas, const sn_tunin
     if (!params || n_replicas < 1 || replica < 0 || replica >= n_replicas) return SN_ERR_INVALID;
     std::vector<ReplicaGeom> geom;
     for (int r = 0; r < n_replicas; r++) {
-        if (params[r].dipole_cutoff != params[0].dipole_cutoff) return SN_ERR_INVALID;
-        if (params[r].X < 1 || params[r].Y < 1 || params[r].Z < 1) return SN_ERR_INVALID;
-        geom.push_back({params[r].X, params[r].Y, params[r].Z});
+        const sn_params &p = params[r];
+        if (p.dipole_cutoff != params[0].dipole_cutoff) return SN_ERR_INVALID;
+        if (p.X < 1 || p.Y < 1 || p.Z < 1) return SN_ERR_INVALID;
+        if ((p.Z == 1) != (params[0].Z == 1)) return SN_ERR_INVALID;
+        if (p.dipole_cutoff < 1 || p.dipole_cutoff > 8) return SN_ERR_UNSUPPORTED;
+        if (p.X <= p.dipole_cutoff || p.Y <= p.dipole_cutoff || (p.Z > 1 && p.Z <= p.dipole_cutoff))
+            return SN_ERR_UNSUPPORTED;
+        geom.push_back({p.X, p.Y, p.Z});
     }
     sn_tuning t{};
     if (tuning) t = *tuning;
