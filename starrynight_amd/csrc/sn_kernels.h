@@ -77,8 +77,10 @@ __device__ __forceinline__ double group_sum_dpp(double v)
 __device__ __forceinline__ int s_gchunk_total(const ShapeDev &sh) { return sh.gsteps_n; }
 
 // BLK threads per workgroup (256 or 512); proposals are drawn in chunks of at
-// most prop_stride (<= kBlock) sites.
-template <int G, int NPL, bool MIXED, int BLK>
+// most prop_stride (<= kBlock) sites.  RS > 0 fixes the stride between the
+// staged x, y, z arrays at compile time (regions of at most RS sites), so the
+// three reads of a neighbour share one address and differ by immediate offsets.
+template <int G, int NPL, bool MIXED, int BLK, int RS = 0>
 __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_tile, int ntiles)
 {
     extern __shared__ double smem[];
@@ -95,11 +97,12 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     const int RX = sh.tx + 2 * hx;
     const int region = RX * RY * RZ;
 
+    const int MR = RS > 0 ? RS : a.max_region;            // stride of the staged arrays
     double *sx = smem;
-    double *sy = sx + a.max_region;
-    double *sz = sy + a.max_region;
-    double *sl = sz + a.max_region;                        // lengths (MIXED only)
-    double *prop = MIXED ? sl + a.max_region : sl;         // 4 x prop_stride: px, py, pz, threshold
+    double *sy = sx + MR;
+    double *sz = sy + MR;
+    double *sl = sz + MR;                                  // lengths (MIXED only)
+    double *prop = MIXED ? sl + MR : sl;                   // 4 x prop_stride: px, py, pz, threshold
     const int PS = a.prop_stride;
     int *s_ri = (int *)(prop + 4 * PS);                    // region index of class sites
     int *s_cls = s_ri + a.max_tile_sites;                  // class starts, then chunk colours
@@ -206,8 +209,8 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
             const int e_next = st + 1 < st1 ? s_gstep[(st + 1) * SW + 1 + group] : -1;
             const int flag_next = st + 1 < st1 ? s_gstep[(st + 1) * SW] : 0;
             {
-                double hx_ = 0.0, hy_ = 0.0, hz_ = 0.0;
-                double gx_ = 0.0, gy_ = 0.0, gz_ = 0.0;
+                double hx_, hy_, hz_;
+                double gx_, gy_, gz_;
 #pragma unroll
                 for (int m = 0; m < NPL; m++) {
                     const int pa = ri + dl[m], pb = ri - dl[m];
@@ -223,10 +226,17 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                     }
                     const double s = nx[m] * mx + ny[m] * my + nz[m] * mz;
                     const double ts = w3[m] * s;
-                    hx_ += w[m] * mx + ts * nx[m];
-                    hy_ += w[m] * my + ts * ny[m];
-                    hz_ += w[m] * mz + ts * nz[m];
-                    if (m == 0) { gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz; }
+                    // the first pair starts the sums (no adds of +0.0)
+                    if (m == 0) {
+                        hx_ = w[m] * mx + ts * nx[m];
+                        hy_ = w[m] * my + ts * ny[m];
+                        hz_ = w[m] * mz + ts * nz[m];
+                        gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz;
+                    } else {
+                        hx_ += w[m] * mx + ts * nx[m];
+                        hy_ += w[m] * my + ts * ny[m];
+                        hz_ += w[m] * mz + ts * nz[m];
+                    }
                 }
                 const double px = prop[pq], py = prop[PS + pq], pz = prop[2 * PS + pq];
                 const double thr = prop[3 * PS + pq];
