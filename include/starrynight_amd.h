@@ -80,6 +80,10 @@ const char *sn_status_string(int status);
  * Replaces direct access to the global lattice[x][y][z] (src/starrynight-config.c:32-36). */
 int sn_set_lattice(sn_ctx *ctx, int32_t replica, const double *xyzl);
 int sn_get_lattice(sn_ctx *ctx, int32_t replica, double *xyzl);
+/* sn_set_lattice from a device buffer (double4[N] on the context's device;
+ * the caller's writes to it must be complete).  Same checks: non-finite
+ * values are refused and the lengths decide uniform or mixed. */
+int sn_set_lattice_device(sn_ctx *ctx, int32_t replica, const void *xyzl_dev);
 /* Device address of replica r's lattice (double4[N] in HBM), for zero-copy
  * consumers on the same device.  Valid until sn_destroy. */
 int sn_lattice_device_ptr(sn_ctx *ctx, int32_t replica, void **dev_ptr);
@@ -165,6 +169,10 @@ int sn_energy(sn_ctx *ctx, int32_t replica, double *energy, double parts[4]);
  * dE/dEz, dE/ds, dE/dK} where s scales every dipole length (at s = 1).
  * energy (may be NULL) receives the total energy from the same pass. */
 int sn_gradient(sn_ctx *ctx, int32_t replica, double *grad_p, double dparams[6], double *energy);
+/* sn_gradient with grad_dev (double[N][3] on the context's device, may be
+ * NULL) written in place: no host copy of the N x 3 field.  Returns after
+ * grad_dev is written. */
+int sn_gradient_device(sn_ctx *ctx, int32_t replica, void *grad_dev, double dparams[6], double *energy);
 
 /* Sum of dipole vectors of replica r (M = sum_i p_i, vacancies included as the
  * reference's landau_order / polarisation do, src/starrynight-analysis.c:48-62,

@@ -20,6 +20,7 @@ STATUS = {0: "SN_OK", 1: "SN_ERR_INVALID", 2: "SN_ERR_HIP", 3: "SN_ERR_NOMEM", 4
 # every entry point declared in include/starrynight_amd.h
 EXPORTS = [
     "sn_create", "sn_destroy", "sn_last_error", "sn_status_string", "sn_set_lattice",
+    "sn_set_lattice_device", "sn_gradient_device",
     "sn_get_lattice", "sn_lattice_device_ptr", "sn_init_lattice_random", "sn_set_params",
     "sn_site_energy_batch", "sn_site_energy_batch_device", "sn_mc_sweeps", "sn_sync",
     "sn_mc_replay", "sn_counters", "sn_reset_counters", "sn_energy", "sn_gradient", "sn_moment",
@@ -57,6 +58,15 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is not built; run `python -m starrynight_amd.build` "
                 "(the HIP library is required, there is no CPU fallback)")
+        # torch ships a HIP runtime with the same soname as ROCm's: whichever
+        # loads first serves both.  torch set up after this library's runtime
+        # has reported no GPUs, so torch (when installed) loads first and the
+        # two share its runtime, which also lets device pointers pass between
+        # them (the *_device entry points).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         P, vp = C.POINTER, C.c_void_p
         dp, ip, u8p, u64p = P(C.c_double), P(C.c_int32), P(C.c_uint8), P(C.c_uint64)
@@ -78,6 +88,8 @@ def lib():
         L.sn_reset_counters.argtypes = [vp]
         L.sn_energy.argtypes = [vp, C.c_int32, dp, dp]
         L.sn_gradient.argtypes = [vp, C.c_int32, dp, dp, dp]
+        L.sn_gradient_device.argtypes = [vp, C.c_int32, vp, dp, dp]
+        L.sn_set_lattice_device.argtypes = [vp, C.c_int32, vp]
         L.sn_moment.argtypes = [vp, C.c_int32, dp]
         L.sn_radial_order.argtypes = [vp, C.c_int32, dp, dp, C.POINTER(C.c_int64)]
         L.sn_dipole_potential.argtypes = [vp, C.c_int32, dp]

@@ -14,7 +14,9 @@ Hamiltonian whose single-site differences are the reference's site_energy
 Forward and backward both run on the GPU in one pass of the adjoint kernel
 (energy, dE/dp_i on every site, dE/d(J, Efield, s)); backward only scales the
 stored gradients by the incoming grad.  K and the geometry are fixed by the
-model's ReplicaParams.  There is no CPU path.
+model's ReplicaParams.  GPU tensors stay in device memory (the lattice is
+assembled and the gradient written there); host tensors are copied in and
+out.  Both run the same GPU kernel: there is no CPU path.
 """
 from __future__ import annotations
 
@@ -57,16 +59,35 @@ class EnergyModel:
             ds = 0.0
         return e, g, np.array([dp[0], dp[1], dp[2], dp[3], ds])
 
+    def evaluate_device(self, p: torch.Tensor, lengths: torch.Tensor, J: float, E, s: float):
+        """evaluate() on GPU tensors: the lattice is assembled and the gradient
+        written in device memory (no host copy of the N-site arrays)."""
+        N = self.rp.N
+        lat = torch.empty((N, 4), dtype=torch.float64, device=p.device)
+        lat[:, :3] = p
+        lat[:, 3] = lengths * s
+        self.ctx.set_params(0, self.rp.T, tuple(float(v) for v in E), float(J))
+        self.ctx.set_lattice_device(lat)
+        g = torch.empty((N, 3), dtype=torch.float64, device=p.device)
+        dp, e = self.ctx.gradient_device(g)
+        ds = dp[4] / s if s != 0.0 else 0.0
+        return e, g, np.array([dp[0], dp[1], dp[2], dp[3], ds])
+
 
 class _LatticeEnergy(torch.autograd.Function):
     @staticmethod
     def forward(ctx, p, lengths, J, E, s, model):
-        pn = p.detach().to("cpu", torch.float64).numpy()
-        ln = lengths.detach().to("cpu", torch.float64).numpy()
         En = E.detach().to("cpu", torch.float64).numpy()
-        e, g, dparams = model.evaluate(pn, ln, float(J), En, float(s))
-        ctx.save_for_backward(torch.from_numpy(g).to(p.device),
-                              torch.from_numpy(dparams).to(p.device))
+        if p.is_cuda:
+            e, g, dparams = model.evaluate_device(p.detach().contiguous(),
+                                                  lengths.detach().to(p.device, torch.float64),
+                                                  float(J), En, float(s))
+        else:
+            pn = p.detach().to("cpu", torch.float64).numpy()
+            ln = lengths.detach().to("cpu", torch.float64).numpy()
+            e, g, dparams = model.evaluate(pn, ln, float(J), En, float(s))
+            g = torch.from_numpy(g)
+        ctx.save_for_backward(g.to(p.device), torch.from_numpy(dparams).to(p.device))
         return torch.tensor(e, dtype=torch.float64, device=p.device)
 
     @staticmethod

@@ -69,6 +69,9 @@ struct sn_ctx {
     size_t partial_cap = 0;
     int max_tile_sites = 0, max_cls = 0, max_chunks = 0;
     int field_region = 0;   // largest staged region of the field kernel (z rows padded to odd)
+    int *d_tile_list = nullptr;          // tile indices grouped by replica (tiled energy kernel)
+    std::vector<int> rep_tile_off;       // R + 1 offsets into d_tile_list
+    bool energy_tiled = false;           // energy / gradient through tile_energy_kernel
     bool can_sweep = false;
     std::string sweep_err;
     std::vector<unsigned long long> sweeps_done;
@@ -342,7 +345,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if (c->sched.max_region > kGatherRS * 7 / 8 && c->sched.max_region <= kGatherRS &&
         pick_sweep(G, NPL, false, blk, kGatherRS))
         c->rs = kGatherRS;
-    if ((c->tuning.kernel == 2 || !field_ok) && !pick_sweep(G, NPL, false, blk, c->rs))
+    if (c->can_sweep && (c->tuning.kernel == 2 || !field_ok) && !pick_sweep(G, NPL, false, blk, c->rs))
         return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
 
     HIPCHK(c, hipGetDevice(&c->device));
@@ -467,6 +470,23 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         c->sweep_err = "tile + halo does not fit in LDS; use a smaller tile";
     }
     if (!c->can_sweep) return SN_OK;
+    // the tiled energy kernel stages the field kernel's padded region
+    const size_t energy_lds = (size_t)4 * field_region_doubles(c) * sizeof(double);
+    if (pick_energy(c->rc, c->two_d, false) && energy_lds <= 150 * 1024) {
+        std::vector<int> tl;
+        c->rep_tile_off.assign(c->R + 1, 0);
+        for (int r = 0; r < c->R; r++) {
+            c->rep_tile_off[r] = (int)tl.size();
+            for (size_t t = 0; t < c->sched.tiles.size(); t++)
+                if (c->sched.tiles[t].replica == r) tl.push_back((int)t);
+        }
+        c->rep_tile_off[c->R] = (int)tl.size();
+        if ((st = upload(c, &c->d_tile_list, tl))) return st;
+        for (int m = 0; m < 2; m++)
+            HIPCHK(c, hipFuncSetAttribute((const void *)pick_energy(c->rc, c->two_d, m == 1),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)energy_lds));
+        c->energy_tiled = true;
+    }
     for (int m = 0; m < 2; m++) {
         const void *f = c->use_field ? (const void *)pick_field(c->rc, c->two_d, m == 1)
                                      : (const void *)pick_sweep(G, NPL, m == 1, c->blk, c->rs);
@@ -484,7 +504,7 @@ int sn_destroy(sn_ctx *c)
                     c->d_field_steps, c->d_tile_coord, c->d_chunk_col, c->d_gsteps, c->d_gchunk, c->d_pairs,
                     c->d_offs, c->d_sweep,
                     c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins,
-                    c->d_stencil, c->d_obs};
+                    c->d_stencil, c->d_obs, c->d_tile_list};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -517,6 +537,39 @@ int sn_set_lattice(sn_ctx *c, int32_t r, const double *xyzl)
     HIPCHK(c, hipMemcpy(c->d_lat + h.base, xyzl, h.N * sizeof(double4), hipMemcpyHostToDevice));
     h.mixed = mixed;
     h.L0 = mixed ? 0.0 : L0;
+    return upload_rep(c, r);
+}
+
+int sn_set_lattice_device(sn_ctx *c, int32_t r, const void *xyzl_dev)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!xyzl_dev) return fail(c, SN_ERR_INVALID, "null lattice buffer");
+    RepHost &h = c->reps[r];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the caller's writes to xyzl_dev are complete (its streams synchronised);
+    // the source is checked before anything is copied, as sn_set_lattice does
+    const int nblk = (int)std::min<long long>((h.N + 255) / 256, 1024);
+    if ((st = ensure_partials(c, (size_t)nblk * 3))) return st;
+    hipLaunchKernelGGL(lattice_scan_kernel, dim3(nblk), dim3(256), 0, c->stream, (const double4 *)xyzl_dev,
+                       (long long)h.N, c->d_partials);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> part((size_t)nblk * 3);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->d_partials, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double bad = 0.0, lo = INFINITY, hi = -INFINITY;
+    for (int b = 0; b < nblk; b++) {
+        bad += part[3 * b];
+        lo = std::min(lo, part[3 * b + 1]);
+        hi = std::max(hi, part[3 * b + 2]);
+    }
+    if (bad > 0.0) return fail(c, SN_ERR_INVALID, "non-finite value in lattice");
+    HIPCHK(c, hipMemcpyAsync(c->d_lat + h.base, xyzl_dev, h.N * sizeof(double4), hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    h.mixed = lo != hi;
+    h.L0 = h.mixed ? 0.0 : lo;
     return upload_rep(c, r);
 }
 
@@ -846,21 +899,50 @@ int sn_reset_counters(sn_ctx *c)
     return SN_OK;
 }
 
+// Energy parts and parameter derivatives of replica r (kNPart sums into
+// d_result), and the gradient into dg when non-null: the tiled kernel (one
+// workgroup per tile, LDS-staged stencil) where the lattice has a sweep
+// schedule, else the per-site gather kernel.
+static int run_energy(sn_ctx *c, int r, double *dg)
+{
+    static_assert(kEnergyParts == kNPart, "energy kernels share the partial-sum layout");
+    const RepHost &h = c->reps[r];
+    int st, nblk;
+    if (c->energy_tiled) {
+        nblk = c->rep_tile_off[r + 1] - c->rep_tile_off[r];
+        if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
+        EnergyArgs ea;
+        ea.lat = c->d_lat; ea.reps = c->d_reps; ea.tiles = c->d_tiles; ea.shapes = c->d_shapes;
+        ea.tile_list = c->d_tile_list + c->rep_tile_off[r];
+        ea.grad = dg; ea.partials = c->d_partials;
+        ea.region_doubles = field_region_doubles(c);
+        const size_t lds = (size_t)(h.mixed ? 4 : 3) * ea.region_doubles * sizeof(double);
+        hipLaunchKernelGGL(pick_energy(c->rc, c->two_d, h.mixed != 0), dim3(nblk), dim3(kBlock), lds,
+                           c->stream, ea, r);
+    } else {
+        nblk = (int)((h.N + 255) / 256);
+        if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
+        hipLaunchKernelGGL(energy_grad_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
+                           c->d_offs, c->noff, dg, c->d_partials);
+    }
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk, kNPart,
+                       c->d_result);
+    HIPCHK(c, hipGetLastError());
+    return SN_OK;
+}
+
 int sn_gradient(sn_ctx *c, int32_t r, double *grad_p, double dparams[6], double *energy)
 {
     int st = check_rep(c, r);
     if (st) return st;
     const RepHost &h = c->reps[r];
-    int nblk = (int)((h.N + 255) / 256);
-    if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
     double *dg = nullptr;
     if (grad_p) HIPCHK(c, hipMalloc((void **)&dg, h.N * 3 * sizeof(double)));
-    hipLaunchKernelGGL(energy_grad_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
-                       c->d_offs, c->noff, dg, c->d_partials);
-    HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk, kNPart,
-                       c->d_result);
-    HIPCHK(c, hipGetLastError());
+    if ((st = run_energy(c, r, dg))) {
+        if (dg) (void)hipFree(dg);
+        return st;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double res[kNPart];
     HIPCHK(c, hipMemcpy(res, c->d_result, sizeof(res), hipMemcpyDeviceToHost));
@@ -882,19 +964,32 @@ int sn_gradient(sn_ctx *c, int32_t r, double *grad_p, double dparams[6], double 
     return SN_OK;
 }
 
+int sn_gradient_device(sn_ctx *c, int32_t r, void *grad_dev, double dparams[6], double *energy)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if ((st = run_energy(c, r, (double *)grad_dev))) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double res[kNPart];
+    HIPCHK(c, hipMemcpy(res, c->d_result, sizeof(res), hipMemcpyDeviceToHost));
+    if (dparams) for (int i = 0; i < 6; i++) dparams[i] = res[4 + i];
+    if (energy) {
+        double s = 0.0, cmp = 0.0;
+        for (int i = 0; i < 4; i++) {
+            double t = s + res[i];
+            cmp += (std::fabs(s) >= std::fabs(res[i])) ? (s - t) + res[i] : (res[i] - t) + s;
+            s = t;
+        }
+        *energy = s + cmp;
+    }
+    return SN_OK;
+}
+
 int sn_energy(sn_ctx *c, int32_t r, double *energy, double parts[4])
 {
     int st = check_rep(c, r);
     if (st) return st;
-    const RepHost &h = c->reps[r];
-    int nblk = (int)((h.N + 255) / 256);
-    if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
-    hipLaunchKernelGGL(energy_grad_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
-                       c->d_offs, c->noff, (double *)nullptr, c->d_partials);
-    HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk, kNPart,
-                       c->d_result);
-    HIPCHK(c, hipGetLastError());
+    if ((st = run_energy(c, r, nullptr))) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double res[kNPart];
     HIPCHK(c, hipMemcpy(res, c->d_result, sizeof(res), hipMemcpyDeviceToHost));

@@ -251,6 +251,33 @@ __global__ __launch_bounds__(256) void moment_kernel(const double4 *lat, const R
     if (tid < 3) partials[(long long)blockIdx.x * 3 + tid] = red[tid][0];
 }
 
+// Scan of a lattice written from device memory: per workgroup {count of
+// non-finite values, min length, max length} (the checks sn_set_lattice makes
+// on the host).
+__global__ __launch_bounds__(256) void lattice_scan_kernel(const double4 *lat, long long N, double *partials)
+{
+    __shared__ double red[3][256];
+    const int tid = threadIdx.x;
+    double bad = 0.0, lo = INFINITY, hi = -INFINITY;
+    for (long long s = (long long)blockIdx.x * blockDim.x + tid; s < N; s += (long long)gridDim.x * blockDim.x) {
+        const double4 p = lat[s];
+        if (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z) || !isfinite(p.w)) bad += 1.0;
+        lo = fmin(lo, p.w);
+        hi = fmax(hi, p.w);
+    }
+    red[0][tid] = bad; red[1][tid] = lo; red[2][tid] = hi;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) {
+            red[0][tid] += red[0][tid + st];
+            red[1][tid] = fmin(red[1][tid], red[1][tid + st]);
+            red[2][tid] = fmax(red[2][tid], red[2][tid + st]);
+        }
+        __syncthreads();
+    }
+    if (tid < 3) partials[(long long)blockIdx.x * 3 + tid] = red[tid][0];
+}
+
 // Random initial lattice on the device: unit dipole from the counter RNG
 // (stream tag 0xFFFFFFFF in the sweep slot) and a length drawn from the
 // species table (cumulative prevalence walk, as solid_solution does).

@@ -2,6 +2,7 @@
 // template instances are compiled in their own translation units
 // (sn_gather_kernels.hip, sn_field_kernels_rc*.hip) so the build runs in parallel.
 #pragma once
+#include "sn_energy_tiled.h"
 #include "sn_sweep_field.h"
 
 namespace sn {
@@ -26,5 +27,14 @@ FieldFn pick_field_rc2(bool two_d, bool mixed);
 FieldFn pick_field_rc3(bool two_d, bool mixed);
 FieldFn pick_field_rc4(bool two_d, bool mixed);
 FieldFn pick_field_rc5(bool two_d, bool mixed);
+
+typedef void (*EnergyFn)(EnergyArgs, int);
+// tiled energy / adjoint kernel for DipoleCutOff rc (1..5) or null
+EnergyFn pick_energy(int rc, bool two_d, bool mixed);
+EnergyFn pick_energy_rc1(bool two_d, bool mixed);
+EnergyFn pick_energy_rc2(bool two_d, bool mixed);
+EnergyFn pick_energy_rc3(bool two_d, bool mixed);
+EnergyFn pick_energy_rc4(bool two_d, bool mixed);
+EnergyFn pick_energy_rc5(bool two_d, bool mixed);
 
 }  // namespace sn

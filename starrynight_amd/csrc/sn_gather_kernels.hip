@@ -24,6 +24,18 @@ SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int rs)
     return nullptr;
 }
 
+EnergyFn pick_energy(int rc, bool two_d, bool mixed)
+{
+    switch (rc) {
+    case 1: return pick_energy_rc1(two_d, mixed);
+    case 2: return pick_energy_rc2(two_d, mixed);
+    case 3: return pick_energy_rc3(two_d, mixed);
+    case 4: return pick_energy_rc4(two_d, mixed);
+    case 5: return pick_energy_rc5(two_d, mixed);
+    }
+    return nullptr;
+}
+
 FieldFn pick_field(int rc, bool two_d, bool mixed)
 {
     switch (rc) {

@@ -20,6 +20,18 @@ import numpy as np
 from ._lib import Params, SnError, Tuning, _ptr, lib
 
 
+def _check_device_tensor(t, shape):
+    if not getattr(t, "is_cuda", False):
+        raise TypeError("expected a GPU tensor")
+    if str(t.dtype) != "torch.float64" or tuple(t.shape) != shape or not t.is_contiguous():
+        raise ValueError(f"expected a contiguous float64 tensor of shape {list(shape)}")
+
+
+def _sync_tensor_stream(t):
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
+
+
 @dataclass
 class ReplicaParams:
     """One replica's model (the reference's globals, src/starrynight-config.c:12-93)."""
@@ -120,6 +132,13 @@ class StarryNight:
             raise ValueError(f"lattice must be [{self.replicas[replica].N}, 4]")
         self._chk(lib().sn_set_lattice(self._h, replica, _ptr(a, C.c_double)))
 
+    def set_lattice_device(self, lat, replica: int = 0):
+        """Load replica's lattice from a contiguous float64 [N, 4] torch tensor
+        on the GPU (synchronises the tensor's stream first)."""
+        _check_device_tensor(lat, (self.replicas[replica].N, 4))
+        _sync_tensor_stream(lat)
+        self._chk(lib().sn_set_lattice_device(self._h, replica, C.c_void_p(lat.data_ptr())))
+
     def get_lattice(self, replica: int = 0) -> np.ndarray:
         out = np.empty((self.replicas[replica].N, 4), np.float64)
         self._chk(lib().sn_get_lattice(self._h, replica, _ptr(out, C.c_double)))
@@ -207,6 +226,19 @@ class StarryNight:
         self._chk(lib().sn_gradient(self._h, replica, _ptr(g, C.c_double) if want_field else None,
                                     _ptr(dp, C.c_double), C.byref(e)))
         return g, dp, e.value
+
+    def gradient_device(self, grad, replica: int = 0):
+        """gradient() with dE/dp written into `grad` (contiguous float64 [N, 3]
+        torch tensor on the GPU, or None); returns (dparams, energy)."""
+        ptr = None
+        if grad is not None:
+            _check_device_tensor(grad, (self.replicas[replica].N, 3))
+            _sync_tensor_stream(grad)
+            ptr = C.c_void_p(grad.data_ptr())
+        dp = np.empty(6, np.float64)
+        e = C.c_double(0)
+        self._chk(lib().sn_gradient_device(self._h, replica, ptr, _ptr(dp, C.c_double), C.byref(e)))
+        return dp, e.value
 
     def moment(self, replica: int = 0) -> np.ndarray:
         M = np.empty(3, np.float64)

@@ -9,19 +9,41 @@ from tests.helpers import oracle_model, print_case, rand_lattice
 
 pytestmark = pytest.mark.gpu
 
+# Lattices that have a sweep schedule take the tiled (LDS-staged) energy
+# kernel for DipoleCutOff 1..5; 14^3 at rc = 4 has none and 20^3 at rc = 6
+# is beyond the tiled kernel's cutoffs: both take the per-site gather kernel.
+# `L` is the uniform dipole length (None: mixed lengths with vacancies).
 CASES = [
-    dict(X=12, Y=12, Z=12, DipoleCutOff=3, CageStrain=0.8, Efield=(0.1, -0.2, 0.3), K=0.0),
-    dict(X=16, Y=16, Z=16, DipoleCutOff=3, CageStrain=1.0, Efield=(0.0, 0.0, 0.05), K=0.4),
-    dict(X=25, Y=25, Z=1, DipoleCutOff=3, CageStrain=1.0, Efield=(0.0, 0.0, 0.0), K=0.0),
-    dict(X=14, Y=14, Z=14, DipoleCutOff=4, CageStrain=0.5, Efield=(0.0, 0.02, 0.0), K=0.0),
+    dict(X=12, Y=12, Z=12, DipoleCutOff=3, CageStrain=0.8, Efield=(0.1, -0.2, 0.3), K=0.0, L=None),
+    dict(X=16, Y=16, Z=16, DipoleCutOff=3, CageStrain=1.0, Efield=(0.0, 0.0, 0.05), K=0.4, L=None),
+    dict(X=25, Y=25, Z=1, DipoleCutOff=3, CageStrain=1.0, Efield=(0.0, 0.0, 0.0), K=0.0, L=None),
+    dict(X=14, Y=14, Z=14, DipoleCutOff=4, CageStrain=0.5, Efield=(0.0, 0.02, 0.0), K=0.0, L=None),
+    dict(X=16, Y=16, Z=16, DipoleCutOff=3, CageStrain=1.2, Efield=(0.03, 0.0, -0.01), K=0.0, L=1.0),
+    dict(X=16, Y=16, Z=16, DipoleCutOff=3, CageStrain=0.6, Efield=(0.0, 0.0, 0.0), K=0.3, L=0.8),
+    dict(X=12, Y=16, Z=20, DipoleCutOff=1, CageStrain=1.0, Efield=(0.0, 0.1, 0.0), K=0.0, L=None),
+    dict(X=16, Y=16, Z=16, DipoleCutOff=2, CageStrain=0.7, Efield=(0.0, 0.0, 0.02), K=0.0, L=1.3),
+    dict(X=24, Y=24, Z=24, DipoleCutOff=4, CageStrain=1.0, Efield=(0.0, 0.0, 0.0), K=0.0, L=1.0),
+    dict(X=20, Y=20, Z=22, DipoleCutOff=5, CageStrain=0.9, Efield=(0.01, 0.0, 0.0), K=0.0, L=None),
+    dict(X=20, Y=20, Z=20, DipoleCutOff=6, CageStrain=0.9, Efield=(0.0, 0.0, 0.0), K=0.0, L=None),
+    dict(X=32, Y=24, Z=1, DipoleCutOff=4, CageStrain=1.0, Efield=(0.0, 0.05, 0.0), K=0.2, L=0.9),
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['X']}x{c['Z']}rc{c['DipoleCutOff']}K{c['K']}")
+def _case_lattice(rp, L):
+    lat = rand_lattice(rp.N, np.random.default_rng(rp.N), mixed=L is None)
+    if L is not None:
+        lat[:, 3] = L
+    return lat
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['X']}x{c['Y']}x{c['Z']}rc{c['DipoleCutOff']}"
+                                                      f"K{c['K']}L{c['L']}")
 def test_energy_gradient_parity(sn, oracle, case):
+    case = dict(case)
+    L = case.pop("L")
     rp = sn.ReplicaParams(**case)
-    print_case("energy_gradient", **case)
-    lat = rand_lattice(rp.N, np.random.default_rng(rp.N), mixed=True)
+    print_case("energy_gradient", L=L, **case)
+    lat = _case_lattice(rp, L)
     with sn.StarryNight(rp) as ctx:
         ctx.set_lattice(lat)
         e, parts = ctx.energy()
@@ -122,3 +144,31 @@ def test_recombination_parity(sn, oracle, dims):
         got = ctx.recombination()
     want = oracle_model(oracle, rp).recombination(lat)
     assert np.allclose(got, want, rtol=1e-11, atol=0)
+
+
+def test_device_lattice_and_gradient(sn, oracle):
+    """sn_set_lattice_device / sn_gradient_device give the host entry points'
+    results (uniform lengths detected from device memory), and a non-finite
+    value is refused."""
+    import torch
+    dev = torch.device("cuda:0")
+    for L in (0.8, None):
+        rp = sn.ReplicaParams(16, 16, 16, DipoleCutOff=3, CageStrain=0.9, Efield=(0.0, 0.01, 0.02))
+        print_case("device_gradient", L=L)
+        lat = _case_lattice(rp, L)
+        with sn.StarryNight(rp) as ctx:
+            ctx.set_lattice(lat)
+            g, dp, e = ctx.gradient()
+            ctx.set_lattice(np.zeros_like(lat))
+            ctx.set_lattice_device(torch.tensor(lat, device=dev))
+            gd = torch.empty((rp.N, 3), dtype=torch.float64, device=dev)
+            dpd, ed = ctx.gradient_device(gd)
+            assert np.array_equal(ctx.get_lattice(), lat)
+            assert np.array_equal(gd.cpu().numpy(), g) and np.array_equal(dpd, dp) and ed == e
+            dpn, en = ctx.gradient_device(None)
+            assert np.array_equal(dpn, dp) and en == e
+            bad = torch.tensor(lat, device=dev)
+            bad[77, 1] = float("nan")
+            with pytest.raises(sn.SnError):
+                ctx.set_lattice_device(bad)
+            assert np.array_equal(ctx.get_lattice(), lat)

@@ -56,6 +56,24 @@ def test_untileable_lattice_rejects_sweeps_but_gathers(sn, oracle):
     assert np.allclose(dE, want, rtol=0, atol=1e-10 * max(1.0, np.abs(want).max()))
 
 
+def test_cutoff_beyond_tiles_creates_and_gathers(sn, oracle):
+    """DipoleCutOff = 6 in 3D: no tile + halo fits the LDS region limit, so
+    the context is created without sweeps; site energies still work."""
+    rp = sn.ReplicaParams(20, 20, 20, DipoleCutOff=6, T=300, seed=6)
+    print_case("rc6_no_sweeps", X=20, rc=6)
+    lat = rand_lattice(rp.N, np.random.default_rng(6), mixed=True)
+    with sn.StarryNight([rp]) as ctx:
+        ctx.set_lattice(lat, 0)
+        with pytest.raises(sn.SnError) as e:
+            ctx.MC_sweeps(1)
+        assert e.value.status == 5
+        sites = np.arange(0, rp.N, 61, dtype=np.int32)
+        props = rand_lattice(len(sites), np.random.default_rng(7), mixed=False)[:, :3].copy()
+        dE = ctx.site_energy_batch(sites, props, 0)
+    want, _ = oracle_model(oracle, rp).site_energy_batch(lat, sites, props)
+    assert np.allclose(dE, want, rtol=0, atol=1e-12 * max(1.0, np.abs(want).max()) * 100)
+
+
 def run_pair(sn, oracle, reps, lats, tuning, nsweeps):
     tun = sn.TuningParams(**tuning) if tuning else None
     with sn.StarryNight(reps, tun) as ctx:
