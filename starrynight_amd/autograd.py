@@ -100,6 +100,64 @@ class _LatticeEnergy(torch.autograd.Function):
         return gp, None, gJ, gE, gs, None
 
 
+def _beta(T: float) -> float:
+    """beta as the library computes it (src/starrynight-main.c:215):
+    1/((float)T/300) for integral T, 300/T otherwise."""
+    T = float(T)
+    if T <= 0.0:
+        raise ValueError("the susceptibility needs T > 0")
+    if T == int(T) and T < 16777216.0:
+        return 1.0 / (float(np.float32(T)) / 300.0)
+    return 300.0 / T
+
+
+def field_susceptibility(ctx: StarryNight, replica=0, samples: int = 2000,
+                         sweeps_between: int = 2, equilibration: int = 200, blocks: int = 20):
+    """Field-derivative dielectric susceptibility of one replica from the
+    adjoint's field derivatives.
+
+    For every configuration dE/dEfield = sum_i p_i = M exactly (the energy's
+    field term is +sum_i p_i . Efield, as in site_energy,
+    src/starrynight-montecarlo-core.c:120-121), and in equilibrium
+
+        d<M_a>/dE_b = -beta (<M_a M_b> - <M_a><M_b>),
+
+    so chi_ab = -(1/N) d<M_a>/dE_b = beta/N Cov(dE/dE_a, dE/dE_b).  The chain
+    runs `equilibration` sweeps (every replica of the context sweeps), then
+    `samples` samples `sweeps_between` sweeps apart; errors are
+    leave-one-block-out jackknife over `blocks` equal blocks.  `replica` is an
+    index (one result) or a sequence of indices (a list, one per replica,
+    sampled from the same sweeps).  A result is {"chi": (3,3), "chi_err":
+    (3,3), "dE_dEfield": (3,) mean, "dE_dEfield_err": (3,), "beta", "samples"}."""
+    if samples < 2 * blocks or blocks < 2:
+        raise ValueError("need blocks >= 2 and at least two samples per block")
+    reps = [int(replica)] if np.ndim(replica) == 0 else [int(r) for r in replica]
+    betas = [_beta(ctx.replicas[r].T) for r in reps]
+    ctx.MC_sweeps(equilibration)
+    d = np.empty((len(reps), samples, 3))
+    for k in range(samples):
+        ctx.MC_sweeps(sweeps_between)
+        for j, r in enumerate(reps):
+            dp, _ = ctx.gradient_device(None, r)
+            d[j, k] = dp[1:4]
+    nb = samples // blocks
+    out = []
+    for j, r in enumerate(reps):
+        beta, N = betas[j], ctx.replicas[r].N
+        x = d[j, :nb * blocks]
+
+        def chi_of(y):
+            return beta / N * np.cov(y, rowvar=False, bias=True)
+
+        jk = np.array([chi_of(np.delete(x, np.s_[b * nb:(b + 1) * nb], axis=0)) for b in range(blocks)])
+        bm = x.reshape(blocks, nb, 3).mean(1)
+        out.append({"chi": chi_of(x),
+                    "chi_err": np.sqrt((blocks - 1) / blocks * ((jk - jk.mean(0)) ** 2).sum(0)),
+                    "dE_dEfield": x.mean(0), "dE_dEfield_err": bm.std(0, ddof=1) / np.sqrt(blocks),
+                    "beta": beta, "samples": len(x)})
+    return out[0] if np.ndim(replica) == 0 else out
+
+
 def lattice_energy(p: torch.Tensor, lengths: torch.Tensor, J: torch.Tensor, E: torch.Tensor,
                    s: torch.Tensor, model: EnergyModel) -> torch.Tensor:
     """Total lattice energy (FP64) with gradients to p, J, E and s."""

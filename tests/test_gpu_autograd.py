@@ -8,7 +8,10 @@ Bars (all FP64):
   * dE/d(J, Efield, s) match central differences of the GPU total energy
     (h = 1e-6) to 1e-7 relative;
   * the reduced differentiation workload (32 replicas of 48^3, rc = 3) gives
-    every replica's adjoint equal to the oracle's.
+    every replica's adjoint equal to the oracle's;
+  * the field-derivative dielectric susceptibility from the adjoint's
+    dE/dEfield fluctuations equals the finite-field derivative of <dE/dEfield>
+    (the fluctuation-response identity) within 3 standard errors.
 """
 import numpy as np
 import pytest
@@ -112,3 +115,34 @@ def test_differentiation_workload_32x48cube(sn, oracle):
     for i, (g, dp, e) in enumerate(res):
         M = lats[i][:, :3].sum(0)
         assert np.abs(dp[1:4] - M).max() <= 1e-9 * reps[i].N
+
+
+def test_field_susceptibility_fluctuation_response(sn):
+    """chi_ab = beta/N Cov(dE/dE_a, dE/dE_b) at E = 0 (adjoint field
+    derivatives sampled along the chain) against -(1/N) d<M_z>/dE_z by central
+    differences between chains at E_z = +-delta (the same sweeps).  12^3,
+    rc = 3, T = 900 K (beta = 1/3, paraelectric); beta delta = 0.1 keeps the
+    response linear to 0.1%."""
+    from starrynight_amd.autograd import field_susceptibility
+    delta, T = 0.3, 900
+    reps = [sn.ReplicaParams(12, 12, 12, DipoleCutOff=3, T=T, CageStrain=1.0, Efield=(0.0, 0.0, ez),
+                             seed=900 + k) for k, ez in enumerate((0.0, delta, -delta))]
+    print_case("field_susceptibility", X=12, T=T, delta=delta)
+    with sn.StarryNight(reps) as ctx:
+        for r in range(3):
+            ctx.init_lattice_random(r)
+        res = field_susceptibility(ctx, [0, 1, 2], samples=3000, sweeps_between=2, equilibration=300)
+    N = reps[0].N
+    chi, err = res[0]["chi"], res[0]["chi_err"]
+    mp, mm = res[1]["dE_dEfield"][2], res[2]["dE_dEfield"][2]
+    fd = -(mp - mm) / (2 * delta * N)
+    fd_err = np.hypot(res[1]["dE_dEfield_err"][2], res[2]["dE_dEfield_err"][2]) / (2 * delta * N)
+    print(f"chi diag {np.diag(chi)} +- {np.diag(err)}; finite field {fd:.5f} +- {fd_err:.5f}", flush=True)
+    assert abs(chi[2, 2] - fd) <= 3 * np.hypot(err[2, 2], fd_err)
+    # cubic symmetry: equal diagonal, vanishing off-diagonal
+    for a in range(2):
+        assert abs(chi[a, a] - chi[2, 2]) <= 3 * np.hypot(err[a, a], err[2, 2])
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert abs(chi[a, b]) <= 3 * err[a, b] + 1e-3 * chi[2, 2]
+    # the field tilts the mean polarisation against E (field term +p.E)
+    assert mp < 0 < mm and fd > 0
