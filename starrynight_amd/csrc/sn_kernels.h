@@ -174,6 +174,10 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
 
     const unsigned sweep = a.sweep_ctr[tile.replica];
     const double J = rep.J, K = rep.K, E0 = rep.E[0], E1 = rep.E[1], E2 = rep.E[2];
+    // Uniform lengths: the cage-strain sum rides on the nearest-neighbour
+    // pair's weight, h' = h - J / L0^2 g, so the partial is L0^2 dp.h' and no
+    // separate g is accumulated.  Mixed lengths keep g (L_j sits inside h).
+    const double wn0 = (!MIXED && nnf != 0.0 && rep.L0 != 0.0) ? w[0] - J / (rep.L0 * rep.L0) : w[0];
     const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
     unsigned nacc = 0, nrej = 0;
     __syncthreads();
@@ -211,40 +215,51 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
             {
                 double hx_, hy_, hz_;
                 double gx_, gy_, gz_;
+                // every LDS read of the step is issued before the first use, so
+                // the step waits for LDS latency once, not once per pair
+                double ax[NPL], ay[NPL], az[NPL], bx[NPL], by[NPL], bz[NPL], la[NPL], lb[NPL];
+                const double *cx = sx + ri, *cy = sy + ri, *cz = sz + ri, *cl = sl + ri;
 #pragma unroll
                 for (int m = 0; m < NPL; m++) {
-                    const int pa = ri + dl[m], pb = ri - dl[m];
-                    const double ax = sx[pa], ay = sy[pa], az = sz[pa];
-                    const double bx = sx[pb], by = sy[pb], bz = sz[pb];
-                    const double ux = ax + bx, uy = ay + by, uz = az + bz;
-                    double mx, my, mz;
-                    if (MIXED) {
-                        const double la = sl[pa], lb = sl[pb];
-                        mx = la * ax + lb * bx; my = la * ay + lb * by; mz = la * az + lb * bz;
-                    } else {
-                        mx = ux; my = uy; mz = uz;
-                    }
-                    const double s = nx[m] * mx + ny[m] * my + nz[m] * mz;
-                    const double ts = w3[m] * s;
-                    // the first pair starts the sums (no adds of +0.0)
-                    if (m == 0) {
-                        hx_ = w[m] * mx + ts * nx[m];
-                        hy_ = w[m] * my + ts * ny[m];
-                        hz_ = w[m] * mz + ts * nz[m];
-                        gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz;
-                    } else {
-                        hx_ += w[m] * mx + ts * nx[m];
-                        hy_ += w[m] * my + ts * ny[m];
-                        hz_ += w[m] * mz + ts * nz[m];
-                    }
+                    ax[m] = cx[dl[m]]; ay[m] = cy[dl[m]]; az[m] = cz[dl[m]];
+                    bx[m] = cx[-dl[m]]; by[m] = cy[-dl[m]]; bz[m] = cz[-dl[m]];
+                    if (MIXED) { la[m] = cl[dl[m]]; lb[m] = cl[-dl[m]]; }
                 }
                 const double px = prop[pq], py = prop[PS + pq], pz = prop[2 * PS + pq];
                 const double thr = prop[3 * PS + pq];
                 const double ox = sx[ri], oy = sy[ri], oz = sz[ri];
                 const double Li = MIXED ? sl[ri] : rep.L0;
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for (int m = 0; m < NPL; m++) {
+                    const double ux = ax[m] + bx[m], uy = ay[m] + by[m], uz = az[m] + bz[m];
+                    double mx, my, mz;
+                    if (MIXED) {
+                        mx = la[m] * ax[m] + lb[m] * bx[m];
+                        my = la[m] * ay[m] + lb[m] * by[m];
+                        mz = la[m] * az[m] + lb[m] * bz[m];
+                    } else {
+                        mx = ux; my = uy; mz = uz;
+                    }
+                    const double s = nx[m] * mx + ny[m] * my + nz[m] * mz;
+                    const double ts = w3[m] * s;
+                    // the first pair starts the sums (no adds of +0.0); each
+                    // component is two dependent FMAs
+                    if (m == 0) {
+                        hx_ = fma(ts, nx[m], wn0 * mx);
+                        hy_ = fma(ts, ny[m], wn0 * my);
+                        hz_ = fma(ts, nz[m], wn0 * mz);
+                        if (MIXED) { gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz; }
+                    } else {
+                        hx_ = fma(ts, nx[m], fma(w[m], mx, hx_));
+                        hy_ = fma(ts, ny[m], fma(w[m], my, hy_));
+                        hz_ = fma(ts, nz[m], fma(w[m], mz, hz_));
+                    }
+                }
                 const double coef = MIXED ? Li : Li * rep.L0;
                 const double dx = px - ox, dy = py - oy, dz = pz - oz;
-                double partial = coef * (dx * hx_ + dy * hy_ + dz * hz_) - J * (dx * gx_ + dy * gy_ + dz * gz_);
+                double partial = coef * (dx * hx_ + dy * hy_ + dz * hz_);
+                if (MIXED) partial -= J * (dx * gx_ + dy * gy_ + dz * gz_);
                 partial = group_sum_dpp<G>(partial);
                 double dE = partial + (dx * E0 + dy * E1 + dz * E2);
                 if (K > 0.0) dE += -K * (fabs(px) - fabs(ox) + fabs(py) - fabs(oy));
