@@ -184,7 +184,7 @@ static size_t sweep_lds(const sn_ctx *c, bool mixed)
     const int stride = c->rs > 0 ? c->rs : c->sched.max_region;
     size_t dbl = (size_t)((mixed ? 4 : 3) * stride + 4 * c->prop_chunk) * sizeof(double);
     size_t ints = (size_t)(c->max_tile_sites + c->max_cls + 1 + 2 * (c->max_chunks + 1) +
-                           c->max_gsteps * (c->blk / c->G + 1)) * sizeof(int);
+                           (c->max_gsteps + 1) * (c->blk / c->G + 1)) * sizeof(int);
     return dbl + ints;
 }
 

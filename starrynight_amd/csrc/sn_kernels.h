@@ -204,14 +204,16 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         // holds sites of one colour and a colour's last step ends with the
         // barrier.  The next step's entry and flag load with this step's data.
         const int st0 = s_gchunk[ch], st1 = s_gchunk[ch + 1];
-        int e = st0 < st1 ? s_gstep[st0 * SW + 1 + group] : -1;
-        int flag = st0 < st1 ? s_gstep[st0 * SW] : 0;
+        // (the LDS table has one spare row, so the read one past a chunk's last
+        // step stays in the allocation; its value is never used)
+        int e = s_gstep[st0 * SW + 1 + group];
+        int flag = s_gstep[st0 * SW];
         for (int st = st0; st < st1; st++) {
             const bool active = e >= 0;
             const int ri = active ? (e & 0xFFFF) : ri_origin;   // any tile site keeps the reads in range
             const int pq = active ? (e >> 16) : 0;
-            const int e_next = st + 1 < st1 ? s_gstep[(st + 1) * SW + 1 + group] : -1;
-            const int flag_next = st + 1 < st1 ? s_gstep[(st + 1) * SW] : 0;
+            const int e_next = s_gstep[(st + 1) * SW + 1 + group];
+            const int flag_next = s_gstep[(st + 1) * SW];
             {
                 double hx_, hy_, hz_;
                 double gx_, gy_, gz_;
