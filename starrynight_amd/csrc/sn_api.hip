@@ -37,7 +37,6 @@ struct sn_ctx {
     int G = 16, NPL = 4, npairs = 0, noff = 0;
     int blk = 256;   // gather kernel's threads per workgroup
     int prop_chunk = kBlock;   // gather kernel's proposals per chunk (LDS slots)
-    int rs = 0;                // gather kernel's compile-time region stride (0: runtime)
     long long total_sites = 0;
     size_t lds_bytes = 0;
     hipStream_t stream = nullptr;
@@ -181,8 +180,7 @@ static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &
 // dynamic LDS bytes of the per-phase gather kernel
 static size_t sweep_lds(const sn_ctx *c, bool mixed)
 {
-    const int stride = c->rs > 0 ? c->rs : c->sched.max_region;
-    size_t dbl = (size_t)((mixed ? 4 : 3) * stride + 4 * c->prop_chunk) * sizeof(double);
+    size_t dbl = (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * c->prop_chunk) * sizeof(double);
     size_t ints = (size_t)(c->max_tile_sites + c->max_cls + 1 + 2 * (c->max_chunks + 1) +
                            (c->max_gsteps + 1) * (c->blk / c->G + 1)) * sizeof(int);
     return dbl + ints;
@@ -340,12 +338,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     while (NPL < need) NPL <<= 1;
     c->G = G; c->NPL = NPL;
     const bool field_ok = pick_field(c->rc, c->two_d, false) != nullptr;
-    // the fixed-stride variant when the largest region is close to its stride
-    // (a smaller region would waste LDS on the fixed stride)
-    if (c->sched.max_region > kGatherRS * 7 / 8 && c->sched.max_region <= kGatherRS &&
-        pick_sweep(G, NPL, false, blk, kGatherRS))
-        c->rs = kGatherRS;
-    if (c->can_sweep && (c->tuning.kernel == 2 || !field_ok) && !pick_sweep(G, NPL, false, blk, c->rs))
+    if (c->can_sweep && (c->tuning.kernel == 2 || !field_ok) && !pick_sweep(G, NPL, false, blk))
         return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
 
     HIPCHK(c, hipGetDevice(&c->device));
@@ -489,7 +482,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     }
     for (int m = 0; m < 2; m++) {
         const void *f = c->use_field ? (const void *)pick_field(c->rc, c->two_d, m == 1)
-                                     : (const void *)pick_sweep(G, NPL, m == 1, c->blk, c->rs);
+                                     : (const void *)pick_sweep(G, NPL, m == 1, c->blk);
         HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need_lds));
     }
     return SN_OK;
@@ -723,7 +716,7 @@ static int launch_coarse(sn_ctx *c, bool mixed, int first, int n)
         hipLaunchKernelGGL(pick_field(c->rc, c->two_d, mixed), dim3(n), dim3(kBlock), field_lds(c, mixed),
                            c->stream, fa, first, n);
     } else {
-        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed, c->blk, c->rs), dim3(n), dim3(c->blk), sweep_lds(c, mixed),
+        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed, c->blk), dim3(n), dim3(c->blk), sweep_lds(c, mixed),
                            c->stream, sweep_args(c), first, n);
     }
     HIPCHK(c, hipGetLastError());

@@ -13,13 +13,9 @@ typedef void (*FieldFn)(FieldArgs, int, int);
 // 16 lanes share the scatter of one accepted site in the field-update kernel.
 constexpr int kFieldG = 16;
 
-// Compile-time region stride of the gather kernel's fast variants: regions of
-// up to 14^3 sites (an 8^3 tile with DipoleCutOff 3 halos), padded to 8.
-constexpr int kGatherRS = 2752;
-
-// per-phase gather kernel for G lanes per site, NPL pairs per lane, blk
-// threads per workgroup and region stride rs (0: runtime) (or null)
-SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int rs);
+// per-phase gather kernel for G lanes per site, NPL pairs per lane and blk
+// threads per workgroup (or null)
+SweepFn pick_sweep(int G, int NPL, bool mixed, int blk);
 // field-update kernel for DipoleCutOff rc (1..5) or null
 FieldFn pick_field(int rc, bool two_d, bool mixed);
 FieldFn pick_field_rc1(bool two_d, bool mixed);

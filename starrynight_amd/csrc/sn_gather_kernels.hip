@@ -5,21 +5,17 @@
 
 namespace sn {
 
-SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int rs)
+SweepFn pick_sweep(int G, int NPL, bool mixed, int blk)
 {
-#define SN_CASE(g, n, b, r)                                                                              \
-    if (G == g && NPL == n && blk == b && rs == r) return mixed ? (SweepFn)tile_sweep_kernel<g, n, true, b, r>  \
-                                                                : (SweepFn)tile_sweep_kernel<g, n, false, b, r>;
-    SN_CASE(4, 4, 256, 0) SN_CASE(4, 8, 256, 0) SN_CASE(4, 16, 256, 0) SN_CASE(8, 2, 256, 0)
-    SN_CASE(8, 4, 256, 0) SN_CASE(8, 8, 256, 0) SN_CASE(8, 16, 256, 0) SN_CASE(16, 2, 256, 0)
-    SN_CASE(16, 4, 256, 0) SN_CASE(16, 8, 256, 0) SN_CASE(16, 16, 256, 0) SN_CASE(32, 2, 256, 0)
-    SN_CASE(32, 4, 256, 0) SN_CASE(32, 8, 256, 0)
-    SN_CASE(16, 4, 512, 0) SN_CASE(16, 8, 512, 0) SN_CASE(16, 16, 512, 0) SN_CASE(32, 2, 512, 0)
-    SN_CASE(32, 4, 512, 0) SN_CASE(32, 8, 512, 0) SN_CASE(64, 2, 512, 0) SN_CASE(64, 4, 512, 0)
-    // compile-time region stride for regions of up to 14^3 sites (8^3 tiles at DipoleCutOff 3)
-    SN_CASE(16, 2, 256, kGatherRS) SN_CASE(16, 4, 256, kGatherRS) SN_CASE(32, 2, 256, kGatherRS)
-    SN_CASE(32, 4, 256, kGatherRS) SN_CASE(16, 4, 512, kGatherRS) SN_CASE(32, 2, 512, kGatherRS)
-    SN_CASE(64, 2, 512, kGatherRS)
+#define SN_CASE(g, n, b)                                                                        \
+    if (G == g && NPL == n && blk == b) return mixed ? (SweepFn)tile_sweep_kernel<g, n, true, b>  \
+                                                     : (SweepFn)tile_sweep_kernel<g, n, false, b>;
+    SN_CASE(4, 4, 256) SN_CASE(4, 8, 256) SN_CASE(4, 16, 256) SN_CASE(8, 2, 256)
+    SN_CASE(8, 4, 256) SN_CASE(8, 8, 256) SN_CASE(8, 16, 256) SN_CASE(16, 2, 256)
+    SN_CASE(16, 4, 256) SN_CASE(16, 8, 256) SN_CASE(16, 16, 256) SN_CASE(32, 2, 256)
+    SN_CASE(32, 4, 256) SN_CASE(32, 8, 256)
+    SN_CASE(16, 4, 512) SN_CASE(16, 8, 512) SN_CASE(16, 16, 512) SN_CASE(32, 2, 512)
+    SN_CASE(32, 4, 512) SN_CASE(32, 8, 512) SN_CASE(64, 2, 512) SN_CASE(64, 4, 512)
 #undef SN_CASE
     return nullptr;
 }

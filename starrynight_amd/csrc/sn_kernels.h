@@ -77,10 +77,11 @@ __device__ __forceinline__ double group_sum_dpp(double v)
 __device__ __forceinline__ int s_gchunk_total(const ShapeDev &sh) { return sh.gsteps_n; }
 
 // BLK threads per workgroup (256 or 512); proposals are drawn in chunks of at
-// most prop_stride (<= kBlock) sites.  RS > 0 fixes the stride between the
-// staged x, y, z arrays at compile time (regions of at most RS sites), so the
-// three reads of a neighbour share one address and differ by immediate offsets.
-template <int G, int NPL, bool MIXED, int BLK, int RS = 0>
+// most prop_stride (<= kBlock) sites.  The staged x, y, z arrays are read by
+// separate instructions: a read2 of x and y at a stride that is a multiple of
+// the bank cycle puts both halves in one bank (measured: 2.9 conflict cycles
+// per LDS instruction against 1.3, and 9% slower at 8^3 tiles).
+template <int G, int NPL, bool MIXED, int BLK>
 __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_tile, int ntiles)
 {
     extern __shared__ double smem[];
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     const int RX = sh.tx + 2 * hx;
     const int region = RX * RY * RZ;
 
-    const int MR = RS > 0 ? RS : a.max_region;            // stride of the staged arrays
+    const int MR = a.max_region;                           // stride of the staged arrays
     double *sx = smem;
     double *sy = sx + MR;
     double *sz = sy + MR;
