@@ -137,6 +137,10 @@ def main():
                     help="only the headline sweeps (for profiler runs)")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the other configurations (2D, temperature sweep, field ramp, adjoint)")
+    ap.add_argument("--workload", default=None,
+                    help="run only this one of the other configurations and print its figures "
+                         "(for profiler runs): 2d_25x25_rc3, 64cube_rc3_temperature_sweep, "
+                         "128cube_rc4_solid_solution_ramp, 32x48cube_adjoint")
     ap.add_argument("--copy-kernel", action="store_true",
                     help="run only a 64 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
@@ -151,6 +155,10 @@ def main():
     import torch  # noqa: F401  (loaded before the library: one HIP runtime for both)
 
     import starrynight_amd as sn
+
+    if args.workload:
+        print(json.dumps(other_workloads(sn, log, autotune=not args.no_autotune, only=args.workload)))
+        return
 
     L, rc = args.L, args.rc
     rp = sn.ReplicaParams(L, L, L, DipoleCutOff=rc, CageStrain=1.0, T=300, seed=0x5EED)
@@ -253,76 +261,99 @@ def _timed_sweeps(ctx, sweeps, sites):
     return sweeps * sites / (time.perf_counter() - t0)
 
 
-def other_workloads(sn, log, autotune=True):
+def other_workloads(sn, log, autotune=True, only=None):
     """The remaining configurations, each timed briefly (site-updates/s of the
-    sweeps; the adjoint in gradient evaluations/s)."""
+    sweeps; the adjoint in gradient evaluations/s).  `only` names one of them
+    (a profiler run of a single workload)."""
     import torch
-    out = {}
 
     def tuned(reps, ms):
         return sn.autotune(reps, budget_ms=ms, use_cache=False)[0] if autotune else None
 
-    # 2D 25x25x1, rc = 3, T = 300 (the reference's CPU-runnable case): 1000 sweeps
-    rp = sn.ReplicaParams(25, 25, 1, DipoleCutOff=3, T=300, seed=0xDEADBEEF + 300)
-    with sn.StarryNight(rp, tuned(rp, 1500)) as ctx:
-        ctx.init_lattice_random(0)
-        out["2d_25x25_rc3"] = {"site_updates_per_s": _timed_sweeps(ctx, 1000, rp.N), "sweeps": 1000}
-    # 64^3 rc = 3: one replica, and the 0-500 K sweep in 25 K steps batched as 21 replicas
-    rp = sn.ReplicaParams(64, 64, 64, DipoleCutOff=3, T=300, seed=64)
-    with sn.StarryNight(rp, tuned(rp, 2000)) as ctx:
-        ctx.init_lattice_random(0)
-        single = _timed_sweeps(ctx, 20, rp.N)
-    reps = [sn.ReplicaParams(64, 64, 64, DipoleCutOff=3, T=25.0 * k, seed=640 + k) for k in range(21)]
-    with sn.StarryNight(reps, tuned(reps, 3000)) as ctx:
-        for r in range(len(reps)):
-            ctx.init_lattice_random(r)
-        agg = _timed_sweeps(ctx, 10, sum(r.N for r in reps))
-    out["64cube_rc3_temperature_sweep"] = {"single_replica": single, "21_temperatures_batched": agg,
-                                           "temperatures": "0..500 K step 25"}
-    # 128^3 rc = 4 solid solution (Dipoles [1, 0], Prevalence [0.5, 0.5]), 16 replicas
-    # on an E_z ramp: 3 ramp steps of 3 sweeps each (the full ramp is 41 x 500)
-    reps = [sn.ReplicaParams(128, 128, 128, DipoleCutOff=4, T=300, Efield=(0, 0, -0.1), seed=4000 + k)
-            for k in range(16)]
-    with sn.StarryNight(reps, tuned(reps, 4000)) as ctx:
-        for r in range(len(reps)):
-            ctx.init_lattice_random(r, Dipoles=(1.0, 0.0), Prevalence=(0.5, 0.5))
-        ctx.MC_sweeps(1)
-        t0 = time.perf_counter()
-        steps, per = 3, 3
-        for k in range(steps):
+    def square_2d():
+        # 2D 25x25x1, rc = 3, T = 300 (the reference's CPU-runnable case): 1000 sweeps
+        rp = sn.ReplicaParams(25, 25, 1, DipoleCutOff=3, T=300, seed=0xDEADBEEF + 300)
+        with sn.StarryNight(rp, tuned(rp, 1500)) as ctx:
+            ctx.init_lattice_random(0)
+            return {"site_updates_per_s": _timed_sweeps(ctx, 1000, rp.N), "sweeps": 1000}
+
+    def temperature_sweep():
+        # 64^3 rc = 3: one replica, and the 0-500 K sweep in 25 K steps batched as 21 replicas
+        rp = sn.ReplicaParams(64, 64, 64, DipoleCutOff=3, T=300, seed=64)
+        with sn.StarryNight(rp, tuned(rp, 2000)) as ctx:
+            ctx.init_lattice_random(0)
+            single = _timed_sweeps(ctx, 20, rp.N)
+        reps = [sn.ReplicaParams(64, 64, 64, DipoleCutOff=3, T=25.0 * k, seed=640 + k) for k in range(21)]
+        with sn.StarryNight(reps, tuned(reps, 3000)) as ctx:
             for r in range(len(reps)):
-                ctx.set_params(r, 300, (0.0, 0.0, -0.1 + 0.005 * k), 1.0)
-            ctx.MC_sweeps(per, sync=False)
-        ctx.sync()
-        dt = time.perf_counter() - t0
-        n = sum(r.N for r in reps)
-        out["128cube_rc4_solid_solution_ramp"] = {"site_updates_per_s": steps * per * n / dt,
-                                                  "replicas": len(reps), "sweeps": steps * per}
-    # 32 replicas of 48^3, rc = 3: energy + adjoint (local field and parameter derivatives)
-    reps = [sn.ReplicaParams(48, 48, 48, DipoleCutOff=3, T=300, seed=480 + k) for k in range(32)]
-    with sn.StarryNight(reps) as ctx:
-        for r in range(len(reps)):
-            ctx.init_lattice_random(r)
-        grads = [torch.empty((r.N, 3), dtype=torch.float64, device="cuda:0") for r in reps]
-        ctx.gradient_device(grads[0], 0)
-        ctx.gradient(0)
-        passes = 4
-        t0 = time.perf_counter()
-        for _ in range(passes):
+                ctx.init_lattice_random(r)
+            agg = _timed_sweeps(ctx, 10, sum(r.N for r in reps))
+        return {"single_replica": single, "21_temperatures_batched": agg, "temperatures": "0..500 K step 25"}
+
+    def field_ramp():
+        # 128^3 rc = 4 solid solution (Dipoles [1, 0], Prevalence [0.5, 0.5]), 16 replicas
+        # on an E_z ramp: 3 ramp steps of 3 sweeps each (the full ramp is 41 x 500)
+        reps = [sn.ReplicaParams(128, 128, 128, DipoleCutOff=4, T=300, Efield=(0, 0, -0.1), seed=4000 + k)
+                for k in range(16)]
+        tun = tuned(reps, 4000)
+        with sn.StarryNight(reps, tun) as ctx:
             for r in range(len(reps)):
-                ctx.gradient_device(grads[r], r)
-        dt = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        for r in range(len(reps)):
-            ctx.gradient(r)
-        dt_host = time.perf_counter() - t0
-        out["32x48cube_adjoint"] = {"gradients_per_s": passes * len(reps) / dt, "replicas": len(reps),
-                                    "host_copy_gradients_per_s": len(reps) / dt_host,
-                                    "note": "each gradient: dE/dp on 110592 sites written in HBM + "
-                                            "dE/d(J, E, s, K); host_copy_*: with the N x 3 field copied "
-                                            "to host memory"}
-    for k, v in out.items():
-        log(f"[bench] {k}: {v}")
+                ctx.init_lattice_random(r, Dipoles=(1.0, 0.0), Prevalence=(0.5, 0.5))
+            ctx.MC_sweeps(1)
+            t0 = time.perf_counter()
+            steps, per = 3, 3
+            for k in range(steps):
+                for r in range(len(reps)):
+                    ctx.set_params(r, 300, (0.0, 0.0, -0.1 + 0.005 * k), 1.0)
+                ctx.MC_sweeps(per, sync=False)
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            n = sum(r.N for r in reps)
+            return {"site_updates_per_s": steps * per * n / dt, "replicas": len(reps), "sweeps": steps * per,
+                    "schedule": str(tun.__dict__) if tun else "built-in"}
+
+    def adjoint():
+        # 32 replicas of 48^3, rc = 3: energy + adjoint (local field and parameter derivatives)
+        reps = [sn.ReplicaParams(48, 48, 48, DipoleCutOff=3, T=300, seed=480 + k) for k in range(32)]
+        with sn.StarryNight(reps) as ctx:
+            for r in range(len(reps)):
+                ctx.init_lattice_random(r)
+            g_all = torch.empty((sum(r.N for r in reps), 3), dtype=torch.float64, device="cuda:0")
+            grads = [torch.empty((r.N, 3), dtype=torch.float64, device="cuda:0") for r in reps]
+            ctx.gradient_all_device(g_all)
+            ctx.gradient_device(grads[0], 0)
+            ctx.gradient(0)
+            passes = 10
+            t0 = time.perf_counter()
+            for _ in range(passes):
+                ctx.gradient_all_device(g_all)
+            dt_all = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            for _ in range(2):
+                for r in range(len(reps)):
+                    ctx.gradient_device(grads[r], r)
+            dt_one = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            for r in range(len(reps)):
+                ctx.gradient(r)
+            dt_host = time.perf_counter() - t0
+            return {"gradients_per_s": passes * len(reps) / dt_all, "replicas": len(reps),
+                    "one_call_per_replica_gradients_per_s": 2 * len(reps) / dt_one,
+                    "host_copy_gradients_per_s": len(reps) / dt_host,
+                    "note": "each gradient: dE/dp on 110592 sites written in HBM + dE/d(J, E, s, K); "
+                            "gradients_per_s: all 32 replicas per call (sn_gradient_all_device); "
+                            "one_call_per_replica: sn_gradient_device per replica; host_copy: sn_gradient, "
+                            "the N x 3 field copied to host memory"}
+
+    table = {"2d_25x25_rc3": square_2d, "64cube_rc3_temperature_sweep": temperature_sweep,
+             "128cube_rc4_solid_solution_ramp": field_ramp, "32x48cube_adjoint": adjoint}
+    if only is not None and only not in table:
+        raise SystemExit(f"unknown workload {only!r}; one of {sorted(table)}")
+    out = {}
+    for name, fn in table.items():
+        if only is None or name == only:
+            out[name] = fn()
+            log(f"[bench] {name}: {out[name]}")
     return out
 
 

@@ -173,6 +173,11 @@ int sn_gradient(sn_ctx *ctx, int32_t replica, double *grad_p, double dparams[6],
  * NULL) written in place: no host copy of the N x 3 field.  Returns after
  * grad_dev is written. */
 int sn_gradient_device(sn_ctx *ctx, int32_t replica, void *grad_dev, double dparams[6], double *energy);
+/* Every replica's adjoint in one pass (one launch over all replicas' tiles).
+ * grad_dev (device, may be NULL) holds 3 doubles per site of all replicas in
+ * lattice order: replica r's rows start after the sites of replicas 0..r-1.
+ * dparams (double[R][6]) and energy (double[R]) may be NULL. */
+int sn_gradient_all_device(sn_ctx *ctx, void *grad_dev, double *dparams, double *energy);
 
 /* Sum of dipole vectors of replica r (M = sum_i p_i, vacancies included as the
  * reference's landau_order / polarisation do, src/starrynight-analysis.c:48-62,

@@ -69,6 +69,8 @@ struct sn_ctx {
     int max_tile_sites = 0, max_cls = 0, max_chunks = 0;
     int field_region = 0;   // largest staged region of the field kernel (z rows padded to odd)
     int *d_tile_list = nullptr;          // tile indices grouped by replica (tiled energy kernel)
+    int *d_tile_seg = nullptr;           // R + 1 offsets into d_tile_list (device copy)
+    double *h_result = nullptr;          // pinned host copy of d_result
     std::vector<int> rep_tile_off;       // R + 1 offsets into d_tile_list
     bool energy_tiled = false;           // energy / gradient through tile_energy_kernel
     bool can_sweep = false;
@@ -157,6 +159,14 @@ static int ensure_partials(sn_ctx *c, size_t n)
     HIPCHK(c, hipMalloc((void **)&c->d_partials, n * sizeof(double)));
     c->partial_cap = n;
     return SN_OK;
+}
+
+// Column sums of the nblk x width partials table into d_result (at least 256
+// doubles), kReduceCols columns per workgroup.
+static void launch_reduce(sn_ctx *c, int nblk, int width)
+{
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((width + kReduceCols - 1) / kReduceCols), dim3(256), 0,
+                       c->stream, c->d_partials, nblk, width, c->d_result, (const int *)nullptr);
 }
 
 // Largest tile + halo (sites) staged in LDS: 4 doubles per site (mixed lengths)
@@ -447,7 +457,9 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if ((st = upload(c, &c->d_sweep, zeros_u))) return st;
     std::vector<unsigned long long> zeros_c(2 * std::max<size_t>(c->sched.tiles.size(), 1), 0ull);
     if ((st = upload(c, &c->d_counters, zeros_c))) return st;
-    HIPCHK(c, hipMalloc((void **)&c->d_result, 256 * sizeof(double)));
+    const size_t nresult = std::max<size_t>(256, (size_t)c->R * kNPart);
+    HIPCHK(c, hipMalloc((void **)&c->d_result, nresult * sizeof(double)));
+    HIPCHK(c, hipHostMalloc((void **)&c->h_result, nresult * sizeof(double), hipHostMallocDefault));
     c->sweeps_done.assign(c->R, 0);
     c->replay_counts.assign(2 * c->R, 0ull);
 
@@ -475,6 +487,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         }
         c->rep_tile_off[c->R] = (int)tl.size();
         if ((st = upload(c, &c->d_tile_list, tl))) return st;
+        if ((st = upload(c, &c->d_tile_seg, c->rep_tile_off))) return st;
         for (int m = 0; m < 2; m++)
             HIPCHK(c, hipFuncSetAttribute((const void *)pick_energy(c->rc, c->two_d, m == 1),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)energy_lds));
@@ -497,9 +510,10 @@ int sn_destroy(sn_ctx *c)
                     c->d_field_steps, c->d_tile_coord, c->d_chunk_col, c->d_gsteps, c->d_gchunk, c->d_pairs,
                     c->d_offs, c->d_sweep,
                     c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins,
-                    c->d_stencil, c->d_obs, c->d_tile_list};
+                    c->d_stencil, c->d_obs, c->d_tile_list, c->d_tile_seg};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    if (c->h_result) (void)hipHostFree(c->h_result);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SN_OK;
@@ -892,37 +906,77 @@ int sn_reset_counters(sn_ctx *c)
     return SN_OK;
 }
 
-// Energy parts and parameter derivatives of replica r (kNPart sums into
-// d_result), and the gradient into dg when non-null: the tiled kernel (one
-// workgroup per tile, LDS-staged stencil) where the lattice has a sweep
-// schedule, else the per-site gather kernel.
+// Energy parts and parameter derivatives (kNPart sums per replica into
+// d_result) and the gradient into dg when non-null.  Lattices with a sweep
+// schedule and DipoleCutOff <= 5 take the tiled kernel (one workgroup per
+// tile, LDS-staged stencil), the others the per-site gather kernel.
+// r >= 0: replica r alone, dg holds its N x 3 rows.  r < 0: every replica,
+// dg holds all sites' rows in lattice order (replica r from row base_r), and
+// replica r's sums land at d_result + r * kNPart.
 static int run_energy(sn_ctx *c, int r, double *dg)
 {
     static_assert(kEnergyParts == kNPart, "energy kernels share the partial-sum layout");
-    const RepHost &h = c->reps[r];
-    int st, nblk;
+    const bool all = r < 0;
+    int st;
     if (c->energy_tiled) {
-        nblk = c->rep_tile_off[r + 1] - c->rep_tile_off[r];
+        const int t0 = all ? 0 : c->rep_tile_off[r], t1 = all ? c->rep_tile_off[c->R] : c->rep_tile_off[r + 1];
+        const int nblk = t1 - t0;
         if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
+        // one launch: the mixed-length variant serves uniform replicas too
+        const bool mixed = all ? any_mixed(c) : c->reps[r].mixed != 0;
         EnergyArgs ea;
         ea.lat = c->d_lat; ea.reps = c->d_reps; ea.tiles = c->d_tiles; ea.shapes = c->d_shapes;
-        ea.tile_list = c->d_tile_list + c->rep_tile_off[r];
-        ea.grad = dg; ea.partials = c->d_partials;
+        ea.tile_list = c->d_tile_list + t0;
+        ea.grad = dg; ea.grad_origin = all ? 0 : c->reps[r].base;
+        ea.partials = c->d_partials;
         ea.region_doubles = field_region_doubles(c);
-        const size_t lds = (size_t)(h.mixed ? 4 : 3) * ea.region_doubles * sizeof(double);
-        hipLaunchKernelGGL(pick_energy(c->rc, c->two_d, h.mixed != 0), dim3(nblk), dim3(kBlock), lds,
-                           c->stream, ea, r);
-    } else {
-        nblk = (int)((h.N + 255) / 256);
-        if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
-        hipLaunchKernelGGL(energy_grad_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
-                           c->d_offs, c->noff, dg, c->d_partials);
+        const size_t lds = (size_t)(mixed ? 4 : 3) * ea.region_doubles * sizeof(double);
+        hipLaunchKernelGGL(pick_energy(c->rc, c->two_d, mixed), dim3(nblk), dim3(kBlock), lds, c->stream, ea);
+        HIPCHK(c, hipGetLastError());
+        if (all)
+            hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNPart + kReduceCols - 1) / kReduceCols, c->R),
+                               dim3(256), 0, c->stream, c->d_partials, nblk, kNPart, c->d_result, c->d_tile_seg);
+        else
+            launch_reduce(c, nblk, kNPart);
+        HIPCHK(c, hipGetLastError());
+        return SN_OK;
     }
-    HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk, kNPart,
-                       c->d_result);
-    HIPCHK(c, hipGetLastError());
+    // per-site gather kernel, one replica at a time (the partials buffer is
+    // reused in stream order)
+    for (int q = all ? 0 : r; q < (all ? c->R : r + 1); q++) {
+        const RepHost &h = c->reps[q];
+        const int nblk = (int)((h.N + 255) / 256);
+        if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
+        double *g = dg ? dg + 3 * (all ? h.base : 0) : nullptr;
+        hipLaunchKernelGGL(energy_grad_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, q,
+                           c->d_offs, c->noff, g, c->d_partials);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNPart + kReduceCols - 1) / kReduceCols), dim3(256), 0,
+                           c->stream, c->d_partials, nblk, kNPart, c->d_result + (all ? q * kNPart : 0),
+                           (const int *)nullptr);
+        HIPCHK(c, hipGetLastError());
+    }
     return SN_OK;
+}
+
+// n doubles of d_result into the pinned host copy, after the stream's work
+static int fetch_result(sn_ctx *c, size_t n)
+{
+    HIPCHK(c, hipMemcpyAsync(c->h_result, c->d_result, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SN_OK;
+}
+
+// total energy from its four parts, compensated (Neumaier)
+static double energy_total(const double *parts)
+{
+    double s = 0.0, cmp = 0.0;
+    for (int i = 0; i < 4; i++) {
+        const double t = s + parts[i];
+        cmp += (std::fabs(s) >= std::fabs(parts[i])) ? (s - t) + parts[i] : (parts[i] - t) + s;
+        s = t;
+    }
+    return s + cmp;
 }
 
 int sn_gradient(sn_ctx *c, int32_t r, double *grad_p, double dparams[6], double *energy)
@@ -932,28 +986,17 @@ int sn_gradient(sn_ctx *c, int32_t r, double *grad_p, double dparams[6], double 
     const RepHost &h = c->reps[r];
     double *dg = nullptr;
     if (grad_p) HIPCHK(c, hipMalloc((void **)&dg, h.N * 3 * sizeof(double)));
-    if ((st = run_energy(c, r, dg))) {
+    if ((st = run_energy(c, r, dg)) || (st = fetch_result(c, kNPart))) {
         if (dg) (void)hipFree(dg);
         return st;
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    double res[kNPart];
-    HIPCHK(c, hipMemcpy(res, c->d_result, sizeof(res), hipMemcpyDeviceToHost));
     if (grad_p) {
-        HIPCHK(c, hipMemcpy(grad_p, dg, h.N * 3 * sizeof(double), hipMemcpyDeviceToHost));
+        const hipError_t e = hipMemcpy(grad_p, dg, h.N * 3 * sizeof(double), hipMemcpyDeviceToHost);
         (void)hipFree(dg);
+        if (e != hipSuccess) return fail(c, SN_ERR_HIP, std::string("gradient copy: ") + hipGetErrorString(e));
     }
-    if (dparams) for (int i = 0; i < 6; i++) dparams[i] = res[4 + i];
-    if (energy) {
-        // compensated sum of the four parts
-        double s = 0.0, cmp = 0.0;
-        for (int i = 0; i < 4; i++) {
-            double t = s + res[i];
-            cmp += (std::fabs(s) >= std::fabs(res[i])) ? (s - t) + res[i] : (res[i] - t) + s;
-            s = t;
-        }
-        *energy = s + cmp;
-    }
+    if (dparams) for (int i = 0; i < 6; i++) dparams[i] = c->h_result[4 + i];
+    if (energy) *energy = energy_total(c->h_result);
     return SN_OK;
 }
 
@@ -961,19 +1004,21 @@ int sn_gradient_device(sn_ctx *c, int32_t r, void *grad_dev, double dparams[6], 
 {
     int st = check_rep(c, r);
     if (st) return st;
-    if ((st = run_energy(c, r, (double *)grad_dev))) return st;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    double res[kNPart];
-    HIPCHK(c, hipMemcpy(res, c->d_result, sizeof(res), hipMemcpyDeviceToHost));
-    if (dparams) for (int i = 0; i < 6; i++) dparams[i] = res[4 + i];
-    if (energy) {
-        double s = 0.0, cmp = 0.0;
-        for (int i = 0; i < 4; i++) {
-            double t = s + res[i];
-            cmp += (std::fabs(s) >= std::fabs(res[i])) ? (s - t) + res[i] : (res[i] - t) + s;
-            s = t;
-        }
-        *energy = s + cmp;
+    if ((st = run_energy(c, r, (double *)grad_dev)) || (st = fetch_result(c, kNPart))) return st;
+    if (dparams) for (int i = 0; i < 6; i++) dparams[i] = c->h_result[4 + i];
+    if (energy) *energy = energy_total(c->h_result);
+    return SN_OK;
+}
+
+int sn_gradient_all_device(sn_ctx *c, void *grad_dev, double *dparams, double *energy)
+{
+    if (!c) return SN_ERR_INVALID;
+    int st;
+    if ((st = run_energy(c, -1, (double *)grad_dev)) || (st = fetch_result(c, (size_t)c->R * kNPart))) return st;
+    for (int r = 0; r < c->R; r++) {
+        const double *res = c->h_result + (size_t)r * kNPart;
+        if (dparams) for (int i = 0; i < 6; i++) dparams[6 * r + i] = res[4 + i];
+        if (energy) energy[r] = energy_total(res);
     }
     return SN_OK;
 }
@@ -982,20 +1027,9 @@ int sn_energy(sn_ctx *c, int32_t r, double *energy, double parts[4])
 {
     int st = check_rep(c, r);
     if (st) return st;
-    if ((st = run_energy(c, r, nullptr))) return st;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    double res[kNPart];
-    HIPCHK(c, hipMemcpy(res, c->d_result, sizeof(res), hipMemcpyDeviceToHost));
-    if (parts) for (int i = 0; i < 4; i++) parts[i] = res[i];
-    if (energy) {
-        double s = 0.0, cmp = 0.0;
-        for (int i = 0; i < 4; i++) {
-            double t = s + res[i];
-            cmp += (std::fabs(s) >= std::fabs(res[i])) ? (s - t) + res[i] : (res[i] - t) + s;
-            s = t;
-        }
-        *energy = s + cmp;
-    }
+    if ((st = run_energy(c, r, nullptr)) || (st = fetch_result(c, kNPart))) return st;
+    if (parts) for (int i = 0; i < 4; i++) parts[i] = c->h_result[i];
+    if (energy) *energy = energy_total(c->h_result);
     return SN_OK;
 }
 
@@ -1009,8 +1043,7 @@ int sn_moment(sn_ctx *c, int32_t r, double M[3])
     if ((st = ensure_partials(c, (size_t)nblk * 3))) return st;
     hipLaunchKernelGGL(moment_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r, c->d_partials);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk, 3,
-                       c->d_result);
+    launch_reduce(c, nblk, 3);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(M, c->d_result, 3 * sizeof(double), hipMemcpyDeviceToHost));
@@ -1057,8 +1090,7 @@ int sn_radial_order(sn_ctx *c, int32_t r, double fe[81], double afe[81], int64_t
     hipLaunchKernelGGL(radial_order_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
                        c->d_radial, c->d_radial_bins, c->d_partials);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk,
-                       2 * kRadialBins, c->d_result);
+    launch_reduce(c, nblk, 2 * kRadialBins);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double sums[2 * kRadialBins];
@@ -1154,8 +1186,7 @@ int sn_recombination(sn_ctx *c, int32_t r, double out[8])
     hipLaunchKernelGGL(recombination_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_obs, (long long)h.N,
                        0.165 / 5, 1 / (0.025), c->d_partials);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, c->stream, c->d_partials, nblk,
-                       kRecombParts, c->d_result);
+    launch_reduce(c, nblk, kRecombParts);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double v[kRecombParts];

@@ -198,34 +198,59 @@ __global__ __launch_bounds__(256) void energy_grad_kernel(const double4 *lat, co
     if (tid < kNPart) partials[(long long)blockIdx.x * kNPart + tid] = red[tid][0];
 }
 
-// Fixed-order sum of per-block partials (deterministic): one workgroup,
-// Neumaier-compensated per lane, then a compensated tree.
-__global__ __launch_bounds__(256) void reduce_partials_kernel(const double *partials, int nblocks,
-                                                              int width, double *out)
+// Fixed-order sum of per-block partials (deterministic).  Workgroup (g, y)
+// sums columns [g*kReduceCols, (g+1)*kReduceCols) of rows [seg[y], seg[y+1])
+// (all nblocks rows when seg is null) of the nblocks x width table into
+// out[y * width + col]: every
+// lane Neumaier-sums its rows of all its columns (their loads in flight
+// together), a compensated xor butterfly reduces each wave, and lanes 0..
+// kReduceCols-1 combine the four waves in order.
+constexpr int kReduceCols = 16;
+
+__device__ __forceinline__ void neumaier_add(double &s, double &c, double v)
 {
-    __shared__ double s_[256], c_[256];
-    const int tid = threadIdx.x;
-    for (int col = 0; col < width; col++) {
-        double s = 0.0, c = 0.0;
-        for (int b = tid; b < nblocks; b += 256) {
-            double v = partials[(long long)b * width + col];
-            double t = s + v;
-            c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
-            s = t;
+    const double t = s + v;
+    c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+    s = t;
+}
+
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double *partials, int nblocks,
+                                                              int width, double *out, const int *seg)
+{
+    __shared__ double s_w[kReduceCols][4], c_w[kReduceCols][4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col0 = blockIdx.x * kReduceCols;
+    const int ncol = min(kReduceCols, width - col0);
+    const int b0 = seg ? seg[blockIdx.y] : 0, b1 = seg ? seg[blockIdx.y + 1] : nblocks;
+    out += (long long)blockIdx.y * width;
+    double s[kReduceCols], c[kReduceCols];
+#pragma unroll
+    for (int k = 0; k < kReduceCols; k++) { s[k] = 0.0; c[k] = 0.0; }
+    for (int b = b0 + tid; b < b1; b += 256) {
+        const double *row = partials + (long long)b * width + col0;
+#pragma unroll
+        for (int k = 0; k < kReduceCols; k++)
+            if (k < ncol) neumaier_add(s[k], c[k], row[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kReduceCols; k++) {
+        if (k >= ncol) break;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const double v = __shfl_xor(s[k], o, 64), cv = __shfl_xor(c[k], o, 64);
+            c[k] += cv;
+            neumaier_add(s[k], c[k], v);
         }
-        s_[tid] = s; c_[tid] = c;
-        __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
-            if (tid < st) {
-                double a = s_[tid], v = s_[tid + st];
-                double t = a + v;
-                c_[tid] += c_[tid + st] + ((fabs(a) >= fabs(v)) ? (a - t) + v : (v - t) + a);
-                s_[tid] = t;
-            }
-            __syncthreads();
+        if (lane == 0) { s_w[k][wave] = s[k]; c_w[k][wave] = c[k]; }
+    }
+    __syncthreads();
+    if (tid < ncol) {
+        double S = s_w[tid][0], Cc = c_w[tid][0];
+        for (int w = 1; w < 4; w++) {
+            Cc += c_w[tid][w];
+            neumaier_add(S, Cc, s_w[tid][w]);
         }
-        if (tid == 0) out[col] = s_[0] + c_[0];
-        __syncthreads();
+        out[col0 + tid] = S + Cc;
     }
 }
 

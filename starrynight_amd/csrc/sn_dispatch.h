@@ -24,7 +24,7 @@ FieldFn pick_field_rc3(bool two_d, bool mixed);
 FieldFn pick_field_rc4(bool two_d, bool mixed);
 FieldFn pick_field_rc5(bool two_d, bool mixed);
 
-typedef void (*EnergyFn)(EnergyArgs, int);
+typedef void (*EnergyFn)(EnergyArgs);
 // tiled energy / adjoint kernel for DipoleCutOff rc (1..5) or null
 EnergyFn pick_energy(int rc, bool two_d, bool mixed);
 EnergyFn pick_energy_rc1(bool two_d, bool mixed);

@@ -9,8 +9,9 @@
 //   dE/dJ = -1/2 sum p.g,  dE/dE = sum p,  dE/ds = 2 E_dd (lengths scaled by s),
 //   dE/dK = -sum (|p_x| + |p_y|)  (K > 0).
 //
-// One workgroup per tile (every tile of the replica, in one launch: nothing
-// is written to the lattice, so tiles of all coarse colours run together).
+// One workgroup per tile of `tile_list` (every tile of one replica, or of all
+// replicas, in one launch: nothing is written to the lattice, so tiles of all
+// coarse colours run together).
 // The tile and its halo are staged in LDS with the field sweep's padded
 // layout, and every tile site's h and g come from the same compile-time
 // unrolled column stencil (tile_fields), two z-adjacent sites per lane.
@@ -28,14 +29,15 @@ struct EnergyArgs {
     const RepDev *reps;
     const TileDev *tiles;
     const ShapeDev *shapes;
-    const int *tile_list;     // tiles of the replica (indices into `tiles`)
-    double *grad;             // N x 3 or null
+    const int *tile_list;     // the launch's tiles (indices into `tiles`), grouped by replica
+    double *grad;             // 3 per site (row rep.base - grad_origin + site) or null
+    long long grad_origin;    // lattice row of grad's first row
     double *partials;         // kEnergyParts per workgroup
     int region_doubles;       // stride between the staged x, y, z (, length) arrays
 };
 
 template <int RC, bool TWO_D, bool MIXED>
-__global__ __launch_bounds__(kBlock) void tile_energy_kernel(EnergyArgs ea, int replica)
+__global__ __launch_bounds__(kBlock) void tile_energy_kernel(EnergyArgs ea)
 {
     constexpr int HZ = TWO_D ? 0 : RC;
     constexpr int S = TWO_D ? 1 : 2;
@@ -43,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void tile_energy_kernel(EnergyArgs ea, int 
     __shared__ double red[kEnergyParts][kBlock / 64];
     const int tid = threadIdx.x;
     const TileDev tile = ea.tiles[ea.tile_list[blockIdx.x]];
-    const RepDev rep = ea.reps[replica];
+    const RepDev rep = ea.reps[tile.replica];
     const ShapeDev sh = ea.shapes[tile.shape];
     const int RY = sh.ty + 2 * RC, RZ = sh.tz + 2 * HZ, RX = sh.tx + 2 * RC;
     const int region = RX * RY * RZ;
@@ -105,7 +107,8 @@ __global__ __launch_bounds__(kBlock) void tile_energy_kernel(EnergyArgs ea, int 
             gy -= K * ((p[1] > 0.0) - (p[1] < 0.0));
         }
         if (ea.grad) {
-            const long long s = ((long long)(tile.x0 + lx) * rep.Y + (tile.y0 + ly)) * rep.Z + (tile.z0 + lz);
+            const long long s = rep.base - ea.grad_origin +
+                                ((long long)(tile.x0 + lx) * rep.Y + (tile.y0 + ly)) * rep.Z + (tile.z0 + lz);
             ea.grad[3 * s] = gx; ea.grad[3 * s + 1] = gy; ea.grad[3 * s + 2] = gz;
         }
         v[0] += 0.5 * Li * ph;

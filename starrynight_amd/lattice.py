@@ -240,6 +240,21 @@ class StarryNight:
         self._chk(lib().sn_gradient_device(self._h, replica, ptr, _ptr(dp, C.c_double), C.byref(e)))
         return dp, e.value
 
+    def gradient_all_device(self, grad=None):
+        """Every replica's adjoint in one pass.  `grad` (contiguous float64
+        [sum N, 3] torch tensor on the GPU, or None) receives dE/dp of all
+        sites, replica after replica.  Returns (dparams [R, 6], energies [R])."""
+        ptr = None
+        if grad is not None:
+            _check_device_tensor(grad, (sum(r.N for r in self.replicas), 3))
+            _sync_tensor_stream(grad)
+            ptr = C.c_void_p(grad.data_ptr())
+        R = len(self.replicas)
+        dp = np.empty((R, 6), np.float64)
+        e = np.empty(R, np.float64)
+        self._chk(lib().sn_gradient_all_device(self._h, ptr, _ptr(dp, C.c_double), _ptr(e, C.c_double)))
+        return dp, e
+
     def moment(self, replica: int = 0) -> np.ndarray:
         M = np.empty(3, np.float64)
         self._chk(lib().sn_moment(self._h, replica, _ptr(M, C.c_double)))

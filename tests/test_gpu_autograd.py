@@ -104,6 +104,15 @@ def test_differentiation_workload_32x48cube(sn, oracle):
         for i, l in enumerate(lats):
             ctx.set_lattice(l, i)
         res = [ctx.gradient(i) for i in range(len(reps))]
+        # all 32 in one pass (the batched form the workload is timed with)
+        g_all = torch.empty((sum(r.N for r in reps), 3), dtype=torch.float64, device="cuda:0")
+        dp_all, e_all = ctx.gradient_all_device(g_all)
+        g_all = g_all.cpu().numpy()
+    for i, (g, dp, e) in enumerate(res):
+        gi = g_all[i * reps[0].N:(i + 1) * reps[0].N]
+        assert abs(e_all[i] - e) <= 1e-12 * (abs(e) + reps[i].N)
+        assert np.abs(dp_all[i] - dp).max() <= 1e-11 * (np.abs(dp).max() + 1)
+        assert np.abs(gi - g).max() <= 1e-11 * (np.abs(g).max() + 1)
     for i in (0, 1, 13, 31):
         g, dp, e = res[i]
         o = oracle_model(oracle, reps[i])
