@@ -123,14 +123,20 @@ bool split_axis(int L, int rc, int target, int tmax, std::vector<int> &start, st
     start.clear(); size.clear();
     if (L == 1) { start.push_back(0); size.push_back(1); return true; }
     if (target < rc) target = rc;
+    // the even tile count whose mean edge is closest to the target, first
+    // among splits whose largest tile does not exceed the target (a larger
+    // tile would raise every tile's LDS share in a context whose lengths are
+    // not multiples of the target), then among all valid splits
     int best_n = -1;
-    double best_cost = 1e30;
-    for (int n = 2; n <= L; n += 2) {
-        int lo = L / n, hi = (L + n - 1) / n;
-        if (lo < rc) break;
-        if (hi + 2 * rc > L || hi > tmax) continue;
-        double cost = std::fabs((double)L / n - target);
-        if (cost < best_cost - 1e-9) { best_cost = cost; best_n = n; }
+    for (int pass = 0; pass < 2 && best_n < 0; pass++) {
+        double best_cost = 1e30;
+        for (int n = 2; n <= L; n += 2) {
+            int lo = L / n, hi = (L + n - 1) / n;
+            if (lo < rc) break;
+            if (hi + 2 * rc > L || hi > tmax || (pass == 0 && hi > target)) continue;
+            double cost = std::fabs((double)L / n - target);
+            if (cost < best_cost - 1e-9) { best_cost = cost; best_n = n; }
+        }
     }
     if (best_n < 0) return false;
     int base = L / best_n, extra = L % best_n, pos = 0;
