@@ -183,7 +183,6 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     unsigned nacc = 0, nrej = 0;
     __syncthreads();
 
-    const int ri_origin = (hx * RY + hy) * RZ + hz;   // tile-local (0, 0, 0)
     for (int ch = 0; ch < sh.nchunks; ch++) {
         const int c0 = s_chunk[ch], c1 = s_chunk[ch + 1];
         const int s0 = s_cls[c0], s1 = s_cls[c1];
@@ -210,12 +209,15 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         int e = s_gstep[st0 * SW + 1 + group];
         int flag = s_gstep[st0 * SW];
         for (int st = st0; st < st1; st++) {
+            // a group without a site this step (a short colour) skips the step:
+            // its lanes issue no LDS reads (the group's lanes are whole DPP rows,
+            // so the reduction of the active groups is unaffected)
             const bool active = e >= 0;
-            const int ri = active ? (e & 0xFFFF) : ri_origin;   // any tile site keeps the reads in range
-            const int pq = active ? (e >> 16) : 0;
+            const int ri = e & 0xFFFF;
+            const int pq = e >> 16;
             const int e_next = s_gstep[(st + 1) * SW + 1 + group];
             const int flag_next = s_gstep[(st + 1) * SW];
-            {
+            if (active) {
                 double hx_, hy_, hz_;
                 double gx_, gy_, gz_;
                 // every LDS read of the step is issued before the first use, so
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                 double dE = partial + (dx * E0 + dy * E1 + dz * E2);
                 if (K > 0.0) dE += -K * (fabs(px) - fabs(ox) + fabs(py) - fabs(oy));
                 const bool acc = dE < thr;
-                if (active && part == 0 && Li != 0.0) {
+                if (part == 0 && Li != 0.0) {
                     if (acc) {
                         sx[ri] = px; sy[ri] = py; sz[ri] = pz;
                         nacc++;
