@@ -76,9 +76,9 @@ struct sn_ctx {
     bool can_sweep = false;
     std::string sweep_err;
     std::vector<unsigned long long> sweeps_done;
-    // graph of one sweep
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
+    // graphs of one sweep and of kGraphSweeps sweeps (built on first use)
+    hipGraphExec_t graph_exec = nullptr, graph_exec_k = nullptr;
+    hipGraph_t graph = nullptr, graph_k = nullptr;
     int graph_mixed = -1;
     std::string err;
 };
@@ -137,8 +137,10 @@ static void free_graph(sn_ctx *c)
 {
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
-    c->graph = nullptr;
+    if (c->graph_exec_k) (void)hipGraphExecDestroy(c->graph_exec_k);
+    if (c->graph_k) (void)hipGraphDestroy(c->graph_k);
+    c->graph_exec = c->graph_exec_k = nullptr;
+    c->graph = c->graph_k = nullptr;
     c->graph_mixed = -1;
 }
 
@@ -760,17 +762,31 @@ static int enqueue_sweep(sn_ctx *c, std::vector<hipEvent_t> *events)
     return SN_OK;
 }
 
-static int build_graph(sn_ctx *c)
+// Sweeps per replay of the multi-sweep graph: small lattices, whose sweeps
+// take tens of microseconds, would otherwise be bound by one graph launch
+// per sweep on the host.
+constexpr int kGraphSweeps = 16;
+
+// capture `sweeps` consecutive sweeps into *g / *exec
+static int capture_sweeps(sn_ctx *c, int sweeps, hipGraph_t *graph, hipGraphExec_t *exec)
 {
-    free_graph(c);
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int st = enqueue_sweep(c, nullptr);
+    int st = SN_OK;
+    for (int i = 0; i < sweeps && !st; i++) st = enqueue_sweep(c, nullptr);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (st) { if (g) (void)hipGraphDestroy(g); return st; }
     if (e != hipSuccess) return fail(c, SN_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    c->graph = g;
-    HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    *graph = g;
+    HIPCHK(c, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return SN_OK;
+}
+
+static int build_graph(sn_ctx *c)
+{
+    free_graph(c);
+    int st = capture_sweeps(c, 1, &c->graph, &c->graph_exec);
+    if (st) return st;
     c->graph_mixed = any_mixed(c) ? 1 : 0;
     return SN_OK;
 }
@@ -787,7 +803,15 @@ int sn_mc_sweeps(sn_ctx *c, int32_t nsweeps)
             int st = build_graph(c);
             if (st) return st;
         }
-        for (int i = 0; i < nsweeps; i++) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        int left = nsweeps;
+        if (left >= kGraphSweeps) {
+            if (!c->graph_exec_k) {
+                int st = capture_sweeps(c, kGraphSweeps, &c->graph_k, &c->graph_exec_k);
+                if (st) return st;
+            }
+            for (; left >= kGraphSweeps; left -= kGraphSweeps) HIPCHK(c, hipGraphLaunch(c->graph_exec_k, c->stream));
+        }
+        for (; left > 0; left--) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     } else {
         for (int i = 0; i < nsweeps; i++) {
             int st = enqueue_sweep(c, nullptr);

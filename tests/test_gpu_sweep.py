@@ -210,3 +210,21 @@ def test_field_and_gather_kernels_agree(sn):
             out.append((ctx.get_lattice(), ctx.counters()))
     assert np.abs(out[0][0] - out[1][0]).max() <= 1e-12
     assert out[0][1] == out[1][1]
+
+
+def test_multi_sweep_graph_matches_single_sweeps(sn):
+    """MC_sweeps(n) replays a 16-sweep graph n // 16 times and a one-sweep graph
+    for the rest: 37 sweeps at once equal 37 single-sweep calls, bit for bit,
+    counters included."""
+    rp = sn.ReplicaParams(16, 16, 16, T=350, Efield=(0.0, 0.01, 0.0), seed=37)
+    print_case("multi_sweep_graph", X=16, sweeps=37)
+    lat = rand_lattice(rp.N, np.random.default_rng(37), mixed=True)
+    out = []
+    for chunks in ([37], [1] * 37):
+        with sn.StarryNight(rp) as ctx:
+            ctx.set_lattice(lat)
+            for n in chunks:
+                ctx.MC_sweeps(n)
+            out.append((ctx.get_lattice(), ctx.counters()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and out[0][1]["attempted"] == 37 * rp.N
