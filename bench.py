@@ -267,8 +267,8 @@ def other_workloads(sn, log, autotune=True, only=None):
     (a profiler run of a single workload)."""
     import torch
 
-    def tuned(reps, ms):
-        return sn.autotune(reps, budget_ms=ms, use_cache=False)[0] if autotune else None
+    def tuned(reps, ms, **species):
+        return sn.autotune(reps, budget_ms=ms, use_cache=False, **species)[0] if autotune else None
 
     def square_2d():
         # 2D 25x25x1, rc = 3, T = 300 (the reference's CPU-runnable case): 1000 sweeps
@@ -295,7 +295,7 @@ def other_workloads(sn, log, autotune=True, only=None):
         # on an E_z ramp: 3 ramp steps of 3 sweeps each (the full ramp is 41 x 500)
         reps = [sn.ReplicaParams(128, 128, 128, DipoleCutOff=4, T=300, Efield=(0, 0, -0.1), seed=4000 + k)
                 for k in range(16)]
-        tun = tuned(reps, 4000)
+        tun = tuned(reps, 6000, Dipoles=(1.0, 0.0), Prevalence=(0.5, 0.5))
         with sn.StarryNight(reps, tun) as ctx:
             for r in range(len(reps)):
                 ctx.init_lattice_random(r, Dipoles=(1.0, 0.0), Prevalence=(0.5, 0.5))

@@ -243,13 +243,17 @@ int sn_time_sweep_kernel(sn_ctx *ctx, int32_t nsweeps, double *mean_ms, int32_t 
 /* Autotuner (no counterpart in the reference): times candidate schedules
  * (tile edge targets, fine colouring, field-update or gather kernel) for the
  * given replicas on the current device, each as a real context over a random
- * lattice, and returns the fastest in *best (site-updates/s in *best_rate).
- * budget_ms bounds the search (0: try every candidate).  With use_cache != 0
- * a previous result for the same device, DipoleCutOff and replica sizes is
- * returned from the cache file ($SN_TUNE_CACHE, else
+ * lattice drawn with the species table (n_species lengths / prevalences, as
+ * sn_init_lattice_random; n_species = 0: unit lengths), and returns the
+ * fastest in *best (site-updates/s in *best_rate).  Mixed lengths change the
+ * kernels' LDS use, so the table matters.  budget_ms bounds the search (0: try
+ * every candidate).  With use_cache != 0 a previous result for the same
+ * device, DipoleCutOff, replica sizes and uniform / mixed lengths is returned
+ * from the cache file ($SN_TUNE_CACHE, else
  * $HOME/.cache/starrynight_amd/tune.txt) and new results are appended. */
-int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t budget_ms, int32_t use_cache,
-                sn_tuning *best, double *best_rate);
+int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t n_species, const double *lengths,
+                const double *prevalence, int32_t budget_ms, int32_t use_cache, sn_tuning *best,
+                double *best_rate);
 
 #ifdef __cplusplus
 }

@@ -101,7 +101,7 @@ def lib():
         L.sn_stream.argtypes = [vp, P(vp)]
         L.sn_sweep_config.argtypes = [vp, ip, ip, ip, ip]
         L.sn_time_sweep_kernel.argtypes = [vp, C.c_int32, dp, ip]
-        L.sn_autotune.argtypes = [P(Params), C.c_int32, C.c_int32, C.c_int32, P(Tuning), dp]
+        L.sn_autotune.argtypes = [P(Params), C.c_int32, C.c_int32, dp, dp, C.c_int32, C.c_int32, P(Tuning), dp]
         L.sn_reference_initial_lattice.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_char_p, C.c_uint32,
                                                    C.c_int32, dp, dp, dp]
         L.sn_plan_schedule.argtypes = [P(Params), C.c_int32, P(Tuning), C.c_int32, P(C.c_int64), ip,

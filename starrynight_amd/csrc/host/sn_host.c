@@ -263,7 +263,7 @@ int main(int argc, char **argv)
     snc_lookup_bool(&cf, "Autotune", &autotune);
     if (autotune) {
         double rate = 0.0;
-        int ast = sn_autotune(ps, R, 3000, 1, &tun, &rate);
+        int ast = sn_autotune(ps, R, r.nspec, r.len, r.prev, 3000, 1, &tun, &rate);
         if (ast != SN_OK) { fprintf(stderr, "sn_autotune failed: %s\n", sn_status_string(ast)); return 1; }
         fprintf(stderr, "Autotune: tile %d %d %d, colouring %d, kernel %d, %d threads: %.1f MHz in trial\n",
                 tun.tile[0], tun.tile[1], tun.tile[2], tun.fine_colouring, tun.kernel, tun.block_threads,

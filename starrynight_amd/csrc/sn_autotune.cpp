@@ -44,7 +44,7 @@ void make_parent_dirs(const std::string &path)
         if (path[i] == '/') ::mkdir(path.substr(0, i).c_str(), 0755);
 }
 
-std::string config_key(const sn_params *p, int n)
+std::string config_key(const sn_params *p, int n, bool mixed)
 {
     hipDeviceProp_t prop;
     int dev = 0;
@@ -59,7 +59,7 @@ std::string config_key(const sn_params *p, int n)
         geo[g.str()]++;
     }
     std::ostringstream k;
-    k << "v2|" << name << "|rc" << p[0].dipole_cutoff;
+    k << "v3|" << name << "|rc" << p[0].dipole_cutoff << (mixed ? "|mixed" : "|uniform");
     for (auto &kv : geo) k << "|" << kv.first << "*" << kv.second;
     return k.str();
 }
@@ -99,7 +99,8 @@ void cache_store(const std::string &key, const sn_tuning &t, double rate)
 }
 
 // site-updates/s of one candidate, or < 0 when it cannot run
-double time_candidate(const sn_params *p, int n, const sn_tuning &t, double max_ms)
+double time_candidate(const sn_params *p, int n, int nspec, const double *len, const double *prev,
+                      const sn_tuning &t, double max_ms)
 {
     sn_ctx *ctx = nullptr;
     if (sn_create(p, n, &t, &ctx) != SN_OK) {
@@ -110,7 +111,7 @@ double time_candidate(const sn_params *p, int n, const sn_tuning &t, double max_
     for (int r = 0; r < n; r++) sites += (double)p[r].X * p[r].Y * p[r].Z;
     double rate = -1.0;
     bool ok = true;
-    for (int r = 0; r < n && ok; r++) ok = sn_init_lattice_random(ctx, r, 0, nullptr, nullptr) == SN_OK;
+    for (int r = 0; r < n && ok; r++) ok = sn_init_lattice_random(ctx, r, nspec, len, prev) == SN_OK;
     // warm up (graph capture, caches), then time whole sweeps until max_ms
     ok = ok && sn_mc_sweeps(ctx, 2) == SN_OK && sn_sync(ctx) == SN_OK;
     if (ok) {
@@ -133,11 +134,18 @@ double time_candidate(const sn_params *p, int n, const sn_tuning &t, double max_
 
 }  // namespace
 
-extern "C" int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t budget_ms, int32_t use_cache,
-                           sn_tuning *best, double *best_rate)
+extern "C" int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t n_species, const double *lengths,
+                           const double *prevalence, int32_t budget_ms, int32_t use_cache, sn_tuning *best,
+                           double *best_rate)
 {
     if (!params || n_replicas < 1 || !best) return SN_ERR_INVALID;
-    const std::string key = config_key(params, n_replicas);
+    if (n_species < 0 || n_species > 10 || (n_species > 0 && (!lengths || !prevalence))) return SN_ERR_INVALID;
+    // mixed: two species of non-zero prevalence with different lengths
+    bool mixed = false;
+    for (int i = 0; i < n_species; i++)
+        for (int j = 0; j < i; j++)
+            if (prevalence[i] > 0.0 && prevalence[j] > 0.0 && lengths[i] != lengths[j]) mixed = true;
+    const std::string key = config_key(params, n_replicas, mixed);
     double rate = 0.0;
     if (use_cache && cache_lookup(key, best, &rate)) {
         if (best_rate) *best_rate = rate;
@@ -152,11 +160,13 @@ extern "C" int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t 
         tiles = {{8, 8, 8}, {8, 8, 6}, {8, 8, 10}, {6, 8, 8}, {6, 6, 6}, {8, 6, 6}, {4, 8, 8}, {8, 8, 4}, {4, 4, 8}};
     std::vector<int> colourings = (!two_d && rc == 3) ? std::vector<int>{3, 2, 1} : std::vector<int>{2, 1};
     std::vector<sn_tuning> cands;
-    // (kernel, threads per tile): field-update; gather with 4 or 8 waves
+    // (kernel, threads per tile): gather with 8 or 4 waves, field-update.  The
+    // kernels vary fastest, then the tiles (most promising first), so a budget
+    // that ends the search early has tried every kernel on the leading tiles.
     const int kb[3][2] = {{2, 512}, {2, 256}, {1, 0}};
-    for (const auto &v : kb)
-        for (int c : colourings)
-            for (auto &tl : tiles) {
+    for (int c : colourings)
+        for (auto &tl : tiles)
+            for (const auto &v : kb) {
                 sn_tuning t{};
                 t.tile[0] = tl[0]; t.tile[1] = tl[1]; t.tile[2] = tl[2];
                 t.fine_colouring = c;
@@ -169,7 +179,7 @@ extern "C" int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t 
     sn_tuning best_t{};
     const auto t0 = std::chrono::steady_clock::now();
     for (const sn_tuning &t : cands) {
-        const double r = time_candidate(params, n_replicas, t, per);
+        const double r = time_candidate(params, n_replicas, n_species, lengths, prevalence, t, per);
         if (r > best_r) { best_r = r; best_t = t; }
         const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (budget_ms > 0 && el > budget_ms && best_r > 0.0) break;

@@ -234,14 +234,15 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double *part
     }
 #pragma unroll
     for (int k = 0; k < kReduceCols; k++) {
-        if (k >= ncol) break;
+        if (k < ncol) {   // uniform over the workgroup
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            const double v = __shfl_xor(s[k], o, 64), cv = __shfl_xor(c[k], o, 64);
-            c[k] += cv;
-            neumaier_add(s[k], c[k], v);
+            for (int o = 32; o >= 1; o >>= 1) {
+                const double v = __shfl_xor(s[k], o, 64), cv = __shfl_xor(c[k], o, 64);
+                c[k] += cv;
+                neumaier_add(s[k], c[k], v);
+            }
+            if (lane == 0) { s_w[k][wave] = s[k]; c_w[k][wave] = c[k]; }
         }
-        if (lane == 0) { s_w[k][wave] = s[k]; c_w[k][wave] = c[k]; }
     }
     __syncthreads();
     if (tid < ncol) {

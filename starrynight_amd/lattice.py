@@ -368,15 +368,23 @@ def reference_initial_lattice(X: int, Y: int, Z: int, InitialLattice: str = "ran
     return out
 
 
-def autotune(replicas, budget_ms: int = 3000, use_cache: bool = True):
+def autotune(replicas, budget_ms: int = 3000, use_cache: bool = True, Dipoles=None, Prevalence=None):
     """sn_autotune: the fastest measured schedule for these replicas on the
-    current device, as (TuningParams, site-updates/s)."""
+    current device, as (TuningParams, site-updates/s).  Dipoles / Prevalence
+    (the species table the lattices will hold; default unit lengths) are used
+    for the trial lattices: mixed lengths change the kernels' LDS use."""
     if isinstance(replicas, ReplicaParams):
         replicas = [replicas]
     arr = (Params * len(replicas))(*[r.to_c() for r in replicas])
     t = Tuning()
     rate = C.c_double(0)
-    st = lib().sn_autotune(arr, len(replicas), int(budget_ms), int(bool(use_cache)), C.byref(t), C.byref(rate))
+    ln = np.ascontiguousarray(Dipoles if Dipoles is not None else [], np.float64)
+    pv = np.ascontiguousarray(Prevalence if Prevalence is not None else [], np.float64)
+    if len(ln) != len(pv):
+        raise ValueError("Dipoles and Prevalence differ in length")
+    st = lib().sn_autotune(arr, len(replicas), len(ln), _ptr(ln, C.c_double) if len(ln) else None,
+                           _ptr(pv, C.c_double) if len(pv) else None, int(budget_ms), int(bool(use_cache)),
+                           C.byref(t), C.byref(rate))
     if st != 0:
         raise SnError(st, "autotune found no runnable schedule")
     return TuningParams(tile=tuple(t.tile), lanes_per_site=t.lanes_per_site, fine_colouring=t.fine_colouring,
