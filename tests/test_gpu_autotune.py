@@ -44,3 +44,23 @@ def test_autotune_uneven_batch(sn, tmp_path, monkeypatch):
         cnt = ctx.counters(3)
     assert cnt["attempted"] == 2 * 33 ** 3
     assert not (tmp_path / "tune.txt").exists()
+
+
+def test_autotune_species_table_keys_the_cache(sn, tmp_path, monkeypatch):
+    """Trial lattices follow the species table: a mixed table (Dipoles [1, 0],
+    Prevalence [.5, .5]) is tuned and cached apart from unit lengths, and its
+    pick runs a mixed lattice."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("SN_TUNE_CACHE", str(cache))
+    rp = sn.ReplicaParams(24, 24, 24, DipoleCutOff=4, T=300, seed=4)
+    print_case("autotune_species", X=24, rc=4)
+    tun_m, rate_m = sn.autotune(rp, budget_ms=1500, Dipoles=(1.0, 0.0), Prevalence=(0.5, 0.5))
+    tun_u, rate_u = sn.autotune(rp, budget_ms=1500)
+    text = cache.read_text()
+    assert "|mixed|" in text and "|uniform|" in text and rate_m > 0 and rate_u > 0
+    with pytest.raises(ValueError):
+        sn.autotune(rp, Dipoles=(1.0, 0.0), Prevalence=(1.0,))
+    with sn.StarryNight(rp, tun_m) as ctx:
+        ctx.init_lattice_random(0, Dipoles=(1.0, 0.0), Prevalence=(0.5, 0.5))
+        ctx.MC_sweeps(2)
+        assert ctx.counters()["attempted"] == 2 * rp.N
