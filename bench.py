@@ -180,10 +180,14 @@ def main():
     log(f"[bench] {L}^3 rc={rc} N={N} neighbours={n_nb} F={F} sweep config {cfg}")
     ctx.MC_sweeps(args.warmup)
     ctx.reset_counters()
+    # one GPU, one process: the barrier is trivial; the library stream and
+    # torch's device are both synchronised on each side of the timed steps
     ctx.sync()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     ctx.MC_sweeps(args.steps, sync=False)
     ctx.sync()
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
     dt = t1 - t0
     rate = args.steps * N / dt
