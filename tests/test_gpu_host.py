@@ -131,3 +131,20 @@ def test_host_batched_schedule(sn, tmp_path):
     for i in range(6):
         assert (tmp_path / f"rdf_r{i:03d}.dat").exists()
     assert "ACCEPT:" in err
+
+
+def test_host_autotune_with_species(sn, tmp_path, monkeypatch):
+    """Autotune = true tunes with the cfg's Dipoles / Prevalence (mixed lengths)
+    and the run goes on with the pick."""
+    print_case("sn_host", cfg="autotune mixed")
+    if not os.path.exists(HOST):
+        pytest.skip("sn_host not built")
+    monkeypatch.setenv("SN_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    cfg = CFG.replace("CalculatePotential: true", "CalculatePotential: false").replace(
+        "CalculateRecombination: true", "CalculateRecombination: false").replace(
+        "CalculateEfield: true", "CalculateEfield: false").replace(
+        "SavePotentialCube: true", "SavePotentialCube: false").replace(
+        "Dipoles    = [ 1.0, 0.5 ];", "Dipoles    = [ 1.0, 0.0 ];") + "Autotune = true\n"
+    out, err = run_host(tmp_path, cfg)
+    assert "Autotune: tile" in err and err.count("MC Moves (per second):") == 3
+    assert "|mixed|" in (tmp_path / "tune.txt").read_text()
