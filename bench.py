@@ -110,9 +110,12 @@ def cpu_baseline(seconds_target: float = 10.0, procs: int | None = None):
     ps = [subprocess.Popen([exe, "128", "128", "128", "3", str(300 + 10 * i), str(moves)],
                            stdout=subprocess.PIPE) for i in range(procs)]
     rates = [json.loads(p.communicate()[0])["rate"] for p in ps]
+    # the 2D 25x25x1 rc=3 case (the reference's CPU-runnable configuration), one core, 3000 sweeps
+    sq = json.loads(subprocess.check_output([exe, "25", "25", "1", "3", "300", str(3000 * 625)]))
     return {
         "value": sum(rates), "unit": "site-updates/s", "cores": procs, "kind": "port",
         "single_core": single["rate"],
+        "square_25x25_rc3_single_core": sq["rate"],
         "sample": f"128^3 rc=3 T=300 random lattice, {moves} moves per process "
                   f"(~{moves / single['rate']:.0f} s), float-faithful restatement of "
                   f"MC_moves/site_energy with MT19937, gcc -O3, {procs} processes",
