@@ -142,7 +142,7 @@ def main():
                     help="skip the other configurations (2D, temperature sweep, field ramp, adjoint)")
     ap.add_argument("--workload", default=None,
                     help="run only this one of the other configurations and print its figures "
-                         "(for profiler runs): 2d_25x25_rc3, 64cube_rc3_temperature_sweep, "
+                         "(for profiler runs): batched, 2d_25x25_rc3, 64cube_rc3_temperature_sweep, "
                          "128cube_rc4_solid_solution_ramp, 32x48cube_adjoint")
     ap.add_argument("--copy-kernel", action="store_true",
                     help="run only a 64 MiB device copy (profiler counter calibration)")
@@ -159,6 +159,9 @@ def main():
 
     import starrynight_amd as sn
 
+    if args.workload == "batched":
+        print(json.dumps(run_batched(sn, args.rc, log, autotune=not args.no_autotune)))
+        return
     if args.workload:
         print(json.dumps(other_workloads(sn, log, autotune=not args.no_autotune, only=args.workload)))
         return
