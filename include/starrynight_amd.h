@@ -58,7 +58,11 @@ typedef struct sn_tuning {
     int32_t fine_colouring;   /* 0 auto, 1 cubic (rc+1)^3, 2 min-index sublattice, 3 BCC (rc=3) */
     int32_t use_graph;        /* 0 or 1: replay sweeps from a captured hipGraph; -1: direct launches */
     int32_t kernel;           /* 0 auto, 1 field-update tile kernel, 2 per-phase gather tile kernel
-                                 (both visit sites in the same order with the same draws) */
+                                 (both visit sites in the same order with the same draws),
+                                 3 whole-lattice sweeps of small replicas: one workgroup per replica
+                                 holds it in LDS, colours of a periodic sublattice, many sweeps per
+                                 launch (refused when a replica does not fit or no periodic
+                                 colouring exists; sn_schedule_order reports its order) */
     int32_t block_threads;    /* gather kernel's threads per tile: 0 or 256, or 512 */
 } sn_tuning;
 
@@ -232,7 +236,8 @@ int sn_plan_schedule(const sn_params *params, int32_t n_replicas, const sn_tunin
 /* Stream the context's kernels run on (a hipStream_t), for event timing. */
 int sn_stream(sn_ctx *ctx, void **hip_stream);
 /* Launch configuration the sweep uses: tiles per sweep, tile-sweep kernel
- * launches per sweep, lanes per site, fine colours. */
+ * launches per sweep, lanes per site, fine colours (kernel 3: 0 tiles, one
+ * launch, the periodic colours). */
 int sn_sweep_config(sn_ctx *ctx, int32_t *tiles, int32_t *launches_per_sweep,
                     int32_t *lanes_per_site, int32_t *fine_colours);
 /* Time the tile-sweep kernel alone: runs nsweeps sweeps with a HIP event pair

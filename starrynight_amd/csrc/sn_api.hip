@@ -73,6 +73,12 @@ struct sn_ctx {
     double *h_result = nullptr;          // pinned host copy of d_result
     std::vector<int> rep_tile_off;       // R + 1 offsets into d_tile_list
     bool energy_tiled = false;           // energy / gradient through tile_energy_kernel
+    // whole-lattice sweeps of small replicas (kernel 3): one workgroup per replica
+    bool torus = false;
+    Sublattice torus_lat{};
+    int torus_G = 0, torus_NPL = 0, torus_blk = 0, torus_ms = 0, torus_mc = 0, torus_pc = 0;
+    int *d_torus_sites = nullptr, *d_torus_off = nullptr, *d_torus_cs = nullptr;
+    unsigned long long *d_rep_counters = nullptr;   // 2 per replica (torus sweeps)
     bool can_sweep = false;
     std::string sweep_err;
     std::vector<unsigned long long> sweeps_done;
@@ -261,6 +267,75 @@ const char *sn_status_string(int status)
 
 const char *sn_last_error(const sn_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+// Whole-lattice sweeps (kernel 3): every replica in LDS at once, coloured by
+// a sublattice that holds the replica boxes (periodic_sublattice).  Refused
+// when a replica is too large for LDS or no such colouring exists.
+constexpr int kTorusMaxIndex = 128;
+
+static size_t torus_lds(const sn_ctx *c, bool mixed)
+{
+    return (size_t)((mixed ? 4 : 3) * c->torus_ms + 4 * c->torus_pc) * sizeof(double) +
+           (size_t)c->torus_ms * sizeof(int);
+}
+
+static int setup_torus(sn_ctx *c)
+{
+    if (!periodic_sublattice(c->geom, c->rc, kTorusMaxIndex, c->torus_lat))
+        return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: no periodic colouring of these lattice sizes");
+    std::vector<int> sites, off, cs;
+    int mc = 0;
+    for (int r = 0; r < c->R; r++) {
+        std::vector<int32_t> s, st_;
+        torus_classes(c->torus_lat, c->geom[r], s, st_);
+        off.push_back((int)sites.size());
+        const ReplicaGeom &g = c->geom[r];
+        if (g.X > 1023 || g.Y > 1023 || g.Z > 1023)
+            return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: an axis is longer than 1023");
+        for (int32_t si : s) {   // packed coordinates x | y << 10 | z << 20
+            const int z = si % g.Z, xy = si / g.Z, y = xy % g.Y, x = xy / g.Y;
+            sites.push_back(x | (y << 10) | (z << 20));
+        }
+        cs.insert(cs.end(), st_.begin(), st_.end());
+        for (size_t k = 0; k + 1 < st_.size(); k++) mc = std::max(mc, st_[k + 1] - st_[k]);
+        c->torus_ms = std::max<int>(c->torus_ms, (int)c->reps[r].N);
+    }
+    c->torus_mc = std::max(mc, 1);
+    // lanes per site and pairs per lane for the pair count; 1024 threads when a
+    // colour has enough sites to keep them busy
+    const int np = c->npairs;
+    int G, NPL, blk = 256;
+    if (np <= 16) { G = 4; NPL = 4; }
+    else if (np <= 32) { G = 8; NPL = 4; }
+    else if (np <= 64) {
+        if (c->torus_mc * 16 >= 1024) { G = 32; NPL = 2; blk = 1024; }
+        else { G = 16; NPL = 4; }
+    }
+    else if (np <= 128) { G = 64; NPL = 2; blk = 1024; }
+    else return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: DipoleCutOff too large");
+    c->torus_G = G; c->torus_NPL = NPL; c->torus_blk = blk;
+    // proposal slots: whole runs of colours per draw, as many as LDS leaves room
+    // for (at most the largest replica, at least the largest colour)
+    const long long room = (150LL * 1024 / (long long)sizeof(double) - 4LL * c->torus_ms - c->torus_ms / 2) / 4;
+    c->torus_pc = (int)std::max<long long>(c->torus_mc, std::min<long long>(room, c->torus_ms));
+    const size_t lds = torus_lds(c, true);
+    if (lds > 150 * 1024)
+        return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: a replica does not fit in LDS (" +
+                                               std::to_string(c->torus_ms) + " sites)");
+    int st;
+    for (int m = 0; m < 2; m++) {
+        const void *f = (const void *)pick_torus(G, NPL, m == 1, blk);
+        if (!f) return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: no kernel for this cutoff");
+        HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    if ((st = upload(c, &c->d_torus_sites, sites))) return st;
+    if ((st = upload(c, &c->d_torus_off, off))) return st;
+    if ((st = upload(c, &c->d_torus_cs, cs))) return st;
+    std::vector<unsigned long long> z(2 * c->R, 0ull);
+    if ((st = upload(c, &c->d_rep_counters, z))) return st;
+    c->torus = true;
+    return SN_OK;
+}
+
 int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuning, sn_ctx **out)
 {
     if (!out) return SN_ERR_INVALID;
@@ -337,6 +412,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     for (const TileShape &s : c->sched.shapes) max_class = std::max(max_class, s.max_class);
     const int blk = c->tuning.block_threads > 0 ? c->tuning.block_threads : kBlock;
     if (blk != 256 && blk != 512) return fail(c, SN_ERR_INVALID, "block_threads must be 256 or 512");
+    if (c->tuning.kernel < 0 || c->tuning.kernel > 3) return fail(c, SN_ERR_INVALID, "kernel must be 0..3");
     if (blk != kBlock && c->tuning.kernel != 2)
         return fail(c, SN_ERR_INVALID, "block_threads applies to the gather kernel (kernel = 2) only");
     c->blk = blk;
@@ -476,7 +552,12 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         c->can_sweep = false;
         c->sweep_err = "tile + halo does not fit in LDS; use a smaller tile";
     }
-    if (!c->can_sweep) return SN_OK;
+    const bool tiles_ok = c->can_sweep;
+    if (c->tuning.kernel == 3) {
+        if ((st = setup_torus(c))) return st;
+        c->can_sweep = true;
+    }
+    if (!tiles_ok) return SN_OK;
     // the tiled energy kernel stages the field kernel's padded region
     const size_t energy_lds = (size_t)4 * field_region_doubles(c) * sizeof(double);
     if (pick_energy(c->rc, c->two_d, false) && energy_lds <= 150 * 1024) {
@@ -512,7 +593,8 @@ int sn_destroy(sn_ctx *c)
                     c->d_field_steps, c->d_tile_coord, c->d_chunk_col, c->d_gsteps, c->d_gchunk, c->d_pairs,
                     c->d_offs, c->d_sweep,
                     c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins,
-                    c->d_stencil, c->d_obs, c->d_tile_list, c->d_tile_seg};
+                    c->d_stencil, c->d_obs, c->d_tile_list, c->d_tile_seg, c->d_torus_sites,
+                    c->d_torus_off, c->d_torus_cs, c->d_rep_counters};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_result) (void)hipHostFree(c->h_result);
@@ -739,6 +821,39 @@ static int launch_coarse(sn_ctx *c, bool mixed, int first, int n)
     return SN_OK;
 }
 
+// Sweeps per whole-lattice launch (bounds one launch's duration).
+constexpr int kTorusSweeps = 256;
+
+// nsw whole-lattice sweeps of every replica in one launch, then the
+// sweep-counter bump; events (optional) bracket the sweep kernel.
+static int launch_torus(sn_ctx *c, int nsw, std::vector<hipEvent_t> *events)
+{
+    const bool mixed = any_mixed(c);
+    TorusArgs t;
+    t.lat = c->d_lat; t.reps = c->d_reps; t.pairs = c->d_pairs; t.npairs = c->npairs;
+    t.sites = c->d_torus_sites; t.site_off = c->d_torus_off; t.col_start = c->d_torus_cs;
+    t.ncol = c->torus_lat.colours(); t.sweep_ctr = c->d_sweep; t.counters = c->d_rep_counters;
+    t.max_sites = c->torus_ms; t.prop_cap = c->torus_pc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (events) {
+        HIPCHK(c, hipEventCreate(&e0));
+        HIPCHK(c, hipEventCreate(&e1));
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+    }
+    hipLaunchKernelGGL(pick_torus(c->torus_G, c->torus_NPL, mixed, c->torus_blk), dim3(c->R), dim3(c->torus_blk),
+                       torus_lds(c, mixed), c->stream, t, nsw);
+    HIPCHK(c, hipGetLastError());
+    if (events) {
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        events->push_back(e0);
+        events->push_back(e1);
+    }
+    hipLaunchKernelGGL(bump_sweep_kernel, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, c->d_sweep, c->R,
+                       (unsigned)nsw);
+    HIPCHK(c, hipGetLastError());
+    return SN_OK;
+}
+
 static int enqueue_sweep(sn_ctx *c, std::vector<hipEvent_t> *events)
 {
     const bool mixed = any_mixed(c);
@@ -757,7 +872,7 @@ static int enqueue_sweep(sn_ctx *c, std::vector<hipEvent_t> *events)
             events->push_back(e1);
         } else if ((st = launch_coarse(c, mixed, first, n))) return st;
     }
-    hipLaunchKernelGGL(bump_sweep_kernel, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, c->d_sweep, c->R);
+    hipLaunchKernelGGL(bump_sweep_kernel, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, c->d_sweep, c->R, 1u);
     HIPCHK(c, hipGetLastError());
     return SN_OK;
 }
@@ -796,6 +911,14 @@ int sn_mc_sweeps(sn_ctx *c, int32_t nsweeps)
     if (!c) return SN_ERR_INVALID;
     if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     if (nsweeps < 0) return fail(c, SN_ERR_INVALID, "negative sweep count");
+    if (c->torus) {
+        for (int left = nsweeps; left > 0; left -= kTorusSweeps) {
+            int st = launch_torus(c, std::min(left, kTorusSweeps), nullptr);
+            if (st) return st;
+        }
+        for (int r = 0; r < c->R; r++) c->sweeps_done[r] += (unsigned long long)nsweeps;
+        return SN_OK;
+    }
     const bool use_graph = c->tuning.use_graph >= 0;  // 0 (auto) and 1 capture a graph
     if (use_graph) {
         int want = any_mixed(c) ? 1 : 0;
@@ -835,7 +958,7 @@ int sn_time_sweep_kernel(sn_ctx *c, int32_t nsweeps, double *mean_ms, int32_t *l
     if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     std::vector<hipEvent_t> ev;
     int st = SN_OK;
-    for (int i = 0; i < nsweeps && st == SN_OK; i++) st = enqueue_sweep(c, &ev);
+    for (int i = 0; i < nsweeps && st == SN_OK; i++) st = c->torus ? launch_torus(c, 1, &ev) : enqueue_sweep(c, &ev);
     if (st) return st;
     for (int r = 0; r < c->R; r++) c->sweeps_done[r] += (unsigned long long)nsweeps;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -913,6 +1036,13 @@ int sn_counters(sn_ctx *c, int32_t r, uint64_t *accept, uint64_t *reject, uint64
         for (size_t t = 0; t < nt; t++)
             if (c->sched.tiles[t].replica == r) { acc += tc[2 * t]; rej += tc[2 * t + 1]; }
     }
+    if (c->torus) {
+        unsigned long long rc_[2];
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(rc_, c->d_rep_counters + 2 * r, sizeof(rc_), hipMemcpyDeviceToHost));
+        acc += rc_[0];
+        rej += rc_[1];
+    }
     if (accept) *accept = acc;
     if (reject) *reject = rej;
     if (attempted) *attempted = c->sweeps_done[r] * (unsigned long long)c->reps[r].N;
@@ -925,6 +1055,11 @@ int sn_reset_counters(sn_ctx *c)
     std::vector<unsigned long long> z(2 * std::max<size_t>(c->sched.tiles.size(), 1), 0ull);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->d_counters, z.data(), z.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    if (c->d_rep_counters) {
+        std::vector<unsigned long long> zr(2 * c->R, 0ull);
+        HIPCHK(c, hipMemcpy(c->d_rep_counters, zr.data(), zr.size() * sizeof(unsigned long long),
+                            hipMemcpyHostToDevice));
+    }
     std::fill(c->sweeps_done.begin(), c->sweeps_done.end(), 0ull);
     std::fill(c->replay_counts.begin(), c->replay_counts.end(), 0ull);
     return SN_OK;
@@ -1230,7 +1365,8 @@ int sn_schedule_info(sn_ctx *c, int32_t r, int64_t *n_sites, int32_t *n_phases)
     if (st) return st;
     if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     std::vector<int32_t> order, ps;
-    sweep_order(c->sched, c->geom[r], r, order, ps);
+    if (c->torus) torus_classes(c->torus_lat, c->geom[r], order, ps);
+    else sweep_order(c->sched, c->geom[r], r, order, ps);
     if (n_sites) *n_sites = (int64_t)order.size();
     if (n_phases) *n_phases = (int32_t)ps.size() - 1;
     return SN_OK;
@@ -1242,7 +1378,8 @@ int sn_schedule_order(sn_ctx *c, int32_t r, int32_t *order, int32_t *phase_start
     if (st) return st;
     if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     std::vector<int32_t> o, ps;
-    sweep_order(c->sched, c->geom[r], r, o, ps);
+    if (c->torus) torus_classes(c->torus_lat, c->geom[r], o, ps);
+    else sweep_order(c->sched, c->geom[r], r, o, ps);
     if (order) std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
     if (phase_start) std::memcpy(phase_start, ps.data(), ps.size() * sizeof(int32_t));
     return SN_OK;
@@ -1263,9 +1400,10 @@ int sn_sweep_config(sn_ctx *c, int32_t *tiles, int32_t *launches, int32_t *lanes
         if (c->sched.coarse_start[cc + 1] > c->sched.coarse_start[cc]) nl++;
     int maxcol = 0;
     for (const TileShape &s : c->sched.shapes) maxcol = std::max(maxcol, s.ncol);
-    if (tiles) *tiles = (int32_t)c->sched.tiles.size();
+    if (c->torus) { nl = 1; maxcol = c->torus_lat.colours(); }
+    if (tiles) *tiles = c->torus ? 0 : (int32_t)c->sched.tiles.size();
     if (launches) *launches = nl;
-    if (lanes) *lanes = c->G;
+    if (lanes) *lanes = c->torus ? c->torus_G : c->G;
     if (fine) *fine = maxcol;
     return SN_OK;
 }

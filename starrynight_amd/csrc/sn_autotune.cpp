@@ -160,6 +160,12 @@ extern "C" int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t 
         tiles = {{8, 8, 8}, {8, 8, 6}, {8, 8, 10}, {6, 8, 8}, {6, 6, 6}, {8, 6, 6}, {4, 8, 8}, {8, 8, 4}, {4, 4, 8}};
     std::vector<int> colourings = (!two_d && rc == 3) ? std::vector<int>{3, 2, 1} : std::vector<int>{2, 1};
     std::vector<sn_tuning> cands;
+    {
+        // whole-lattice sweeps (small replicas only; refused at creation otherwise)
+        sn_tuning t{};
+        t.kernel = 3;
+        cands.push_back(t);
+    }
     // (kernel, threads per tile): gather with 8 or 4 waves, field-update.  The
     // kernels vary fastest, then the tiles (most promising first), so a budget
     // that ends the search early has tried every kernel on the leading tiles.

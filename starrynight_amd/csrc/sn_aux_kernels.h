@@ -6,10 +6,10 @@
 #include "sn_device.h"
 
 namespace sn {
-__global__ void bump_sweep_kernel(unsigned *sweep_ctr, int R)
+__global__ void bump_sweep_kernel(unsigned *sweep_ctr, int R, unsigned n)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R) sweep_ctr[r] += 1u;
+    if (r < R) sweep_ctr[r] += n;
 }
 
 // Local field pieces of one site by direct gathers from HBM (used by the

@@ -3,6 +3,7 @@
 // (sn_gather_kernels.hip, sn_field_kernels_rc*.hip) so the build runs in parallel.
 #pragma once
 #include "sn_energy_tiled.h"
+#include "sn_torus.h"
 #include "sn_sweep_field.h"
 
 namespace sn {
@@ -16,6 +17,9 @@ constexpr int kFieldG = 16;
 // per-phase gather kernel for G lanes per site, NPL pairs per lane and blk
 // threads per workgroup (or null)
 SweepFn pick_sweep(int G, int NPL, bool mixed, int blk);
+typedef void (*TorusFn)(TorusArgs, int);
+// whole-lattice sweep kernel of small replicas (or null)
+TorusFn pick_torus(int G, int NPL, bool mixed, int blk);
 // field-update kernel for DipoleCutOff rc (1..5) or null
 FieldFn pick_field(int rc, bool two_d, bool mixed);
 FieldFn pick_field_rc1(bool two_d, bool mixed);

@@ -20,6 +20,16 @@ SweepFn pick_sweep(int G, int NPL, bool mixed, int blk)
     return nullptr;
 }
 
+TorusFn pick_torus(int G, int NPL, bool mixed, int blk)
+{
+#define SN_CASE(g, n, b)                                                                          \
+    if (G == g && NPL == n && blk == b) return mixed ? (TorusFn)torus_sweep_kernel<g, n, true, b>   \
+                                                     : (TorusFn)torus_sweep_kernel<g, n, false, b>;
+    SN_CASE(4, 4, 256) SN_CASE(8, 4, 256) SN_CASE(16, 4, 256) SN_CASE(32, 2, 1024) SN_CASE(64, 2, 1024)
+#undef SN_CASE
+    return nullptr;
+}
+
 EnergyFn pick_energy(int rc, bool two_d, bool mixed)
 {
     switch (rc) {

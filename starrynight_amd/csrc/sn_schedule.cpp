@@ -270,4 +270,50 @@ void sweep_order(const Schedule &s, const ReplicaGeom &g, int replica, std::vect
     phase_start.push_back((int32_t)order.size());
 }
 
+bool periodic_sublattice(const std::vector<ReplicaGeom> &reps, int rc, int max_index, Sublattice &out)
+{
+    if (reps.empty()) return false;
+    const bool two_d = reps[0].Z == 1;
+    auto holds_boxes = [&](const Sublattice &s) {
+        const int c0 = s.colour(0, 0, 0);
+        for (const ReplicaGeom &g : reps)
+            if (s.colour(g.X, 0, 0) != c0 || s.colour(0, g.Y, 0) != c0 || s.colour(0, 0, g.Z) != c0) return false;
+        return true;
+    };
+    for (int idx = 1; idx <= max_index; idx++)
+        for (int a = 1; a <= idx; a++) {
+            if (idx % a) continue;
+            for (int d = 1; d <= idx / a; d++) {
+                if ((idx / a) % d) continue;
+                const int f = idx / (a * d);
+                if (two_d && f != 1) continue;
+                for (int b = 0; b < a; b++)
+                    for (int c = 0; c < (two_d ? 1 : a); c++)
+                        for (int e = 0; e < (two_d ? 1 : d); e++) {
+                            Sublattice s{a, b, c, d, e, f};
+                            if (!holds_boxes(s)) continue;
+                            const int mn = two_d ? min_norm2_plane(s) : s.min_norm2();
+                            if (mn > rc * rc) { out = s; return true; }
+                        }
+            }
+        }
+    return false;
+}
+
+void torus_classes(const Sublattice &lat, const ReplicaGeom &g, std::vector<int32_t> &sites,
+                   std::vector<int32_t> &start)
+{
+    const int nc = lat.colours();
+    std::vector<std::vector<int32_t>> by(nc);
+    for (int x = 0; x < g.X; x++)
+        for (int y = 0; y < g.Y; y++)
+            for (int z = 0; z < g.Z; z++) by[lat.colour(x, y, z)].push_back((x * g.Y + y) * g.Z + z);
+    sites.clear();
+    start.assign(1, 0);
+    for (const auto &v : by) {
+        sites.insert(sites.end(), v.begin(), v.end());
+        start.push_back((int32_t)sites.size());
+    }
+}
+
 }  // namespace sn

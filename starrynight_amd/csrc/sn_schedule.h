@@ -88,4 +88,16 @@ bool build_schedule(const std::vector<ReplicaGeom> &reps, int rc, const int targ
 void sweep_order(const Schedule &s, const ReplicaGeom &g, int replica, std::vector<int32_t> &order,
                  std::vector<int32_t> &phase_start);
 
+// Colouring of whole periodic lattices (no tiles): a sublattice that holds
+// every replica's box vectors (X,0,0), (0,Y,0), (0,0,Z) and whose shortest
+// vector is longer than rc, so that sites of one colour are more than rc apart
+// through the periodic boundary as well.  The smallest index up to max_index;
+// false if there is none.
+bool periodic_sublattice(const std::vector<ReplicaGeom> &reps, int rc, int max_index, Sublattice &out);
+
+// The sites of one replica by colour of `lat` (linear index (x*Y+y)*Z+z,
+// ascending within a colour) and the colour starts (colours + 1 entries).
+void torus_classes(const Sublattice &lat, const ReplicaGeom &g, std::vector<int32_t> &sites,
+                   std::vector<int32_t> &start);
+
 }  // namespace sn

@@ -1,0 +1,203 @@
+// sn_torus.h -- whole-lattice sweep of small replicas in one workgroup (gfx950, FP64).
+//
+// A replica small enough for LDS (its x, y, z (, length) arrays at once) is
+// swept by one workgroup, sweep after sweep, inside one launch: no tiles, no
+// halo staging, no launch per coarse colour.  The sites are coloured by a
+// sublattice that holds the replica's box vectors and whose shortest vector is
+// longer than DipoleCutOff, so the sites of one colour are more than rc apart
+// through the periodic boundary and update together; colours follow one
+// another with a workgroup barrier.  Neighbour indices wrap periodically on
+// the fly (the reference's % X wrap, src/starrynight-montecarlo-core.c:97).
+// Per site the arithmetic is the tile kernels': G lanes share a site, each
+// lane takes NPL of the +-r pairs, one scalar per lane is reduced over the
+// group, and the Metropolis threshold and proposal come from the same
+// counter RNG draw of (seed, replica, sweep, site).
+#pragma once
+#include "sn_kernels.h"
+
+namespace sn {
+
+struct TorusArgs {
+    double4 *lat;
+    const RepDev *reps;
+    const PairDev *pairs;
+    int npairs;
+    const int *sites;          // per replica (at site_off[r]): sites by colour, packed x | y << 10 | z << 20
+    const int *site_off;       // R entries
+    const int *col_start;      // per replica: ncol + 1 colour starts (at r * (ncol + 1))
+    int ncol;
+    const unsigned *sweep_ctr;
+    unsigned long long *counters;  // 2 per replica: accepted, rejected (accumulated)
+    int max_sites;             // stride of the staged arrays
+    int prop_cap;              // proposal slots (>= the largest colour): colours are drawn in
+                               // runs of at most this many sites
+};
+
+template <int G, int NPL, bool MIXED, int BLK>
+__global__ __launch_bounds__(BLK) void torus_sweep_kernel(TorusArgs t, int nsweeps)
+{
+    extern __shared__ double smem[];
+    const int tid = threadIdx.x, r = blockIdx.x;
+    const RepDev rep = t.reps[r];
+    const int X = rep.X, Y = rep.Y, Z = rep.Z, N = X * Y * Z;
+    const int MS = t.max_sites, PS = t.prop_cap;
+    double *sx = smem, *sy = sx + MS, *sz = sy + MS, *sl = sz + MS;
+    double *prop = MIXED ? sl + MS : sl;    // 4 x PS: px, py, pz, threshold of the colour's sites
+    int *sites = (int *)(prop + 4 * PS);    // the replica's site table (MS ints)
+    const int *cs = t.col_start + r * (t.ncol + 1);
+    double4 *lat = t.lat + rep.base;
+
+    // ---- the whole replica and its site table into LDS ----
+    const int *gsites = t.sites + t.site_off[r];
+    for (int i = tid; i < N; i += BLK) {
+        const double4 v = lat[i];
+        sx[i] = v.x; sy[i] = v.y; sz[i] = v.z;
+        if (MIXED) sl[i] = v.w;
+        sites[i] = gsites[i];
+    }
+
+    // ---- this lane's share of the +-r pairs ----
+    const int part = tid % G, group = tid / G;
+    constexpr int NG = BLK / G;
+    int pdx[NPL], pdy[NPL], pdz[NPL];
+    double nx[NPL], ny[NPL], nz[NPL], w[NPL], w3[NPL];
+#pragma unroll
+    for (int m = 0; m < NPL; m++) {
+        const int j = part + m * G;
+        if (j < t.npairs) {
+            const PairDev p = t.pairs[j];
+            pdx[m] = p.dx; pdy[m] = p.dy; pdz[m] = p.dz;
+            nx[m] = p.nx; ny[m] = p.ny; nz[m] = p.nz; w[m] = p.w; w3[m] = -3.0 * p.w;
+        } else {
+            pdx[m] = 0; pdy[m] = 0; pdz[m] = 0;
+            nx[m] = 0.0; ny[m] = 0.0; nz[m] = 0.0; w[m] = 0.0; w3[m] = 0.0;
+        }
+    }
+    // nearest-neighbour pairs come first in the table, so they sit in slot 0
+    const double nnf = (part < t.npairs && t.pairs[part].nn) ? 1.0 : 0.0;
+    const double J = rep.J, K = rep.K, E0 = rep.E[0], E1 = rep.E[1], E2 = rep.E[2];
+    // uniform lengths: the cage-strain sum rides on the nearest-neighbour weight
+    const double wn0 = (!MIXED && nnf != 0.0 && rep.L0 != 0.0) ? w[0] - J / (rep.L0 * rep.L0) : w[0];
+    unsigned nacc = 0, nrej = 0;
+    __syncthreads();
+
+    for (int s = 0; s < nsweeps; s++) {
+        const unsigned sweep = t.sweep_ctr[r] + (unsigned)s;
+        for (int k0 = 0; k0 < t.ncol;) {
+            // ---- proposals of a run of colours [k0, k1) holding at most PS sites ----
+            int k1 = k0 + 1;
+            while (k1 < t.ncol && cs[k1 + 1] - cs[k0] <= PS) k1++;
+            const int p0 = cs[k0];
+            for (int i = tid; i < cs[k1] - p0; i += BLK) {
+                double px, py, pz, u;
+                const int pk = sites[p0 + i];
+                const int si = ((pk & 1023) * Y + ((pk >> 10) & 1023)) * Z + (pk >> 20);
+                counter_draw(rep.seed, (uint32_t)r, sweep, (unsigned)si, rep.constrain_to_x, px, py, pz, u);
+                prop[i] = px;
+                prop[PS + i] = py;
+                prop[2 * PS + i] = pz;
+                prop[3 * PS + i] = metropolis_threshold(u, rep.beta);
+            }
+            __syncthreads();
+          for (int k = k0; k < k1; k++) {
+            const int c0 = cs[k], c1 = cs[k + 1];
+            // ---- the colour's sites, NG at a time (independent of each other) ----
+            for (int b0 = c0; b0 < c1; b0 += NG) {
+                const int q = b0 + group;
+                if (q < c1) {   // whole groups (DPP rows) are active or not
+                    const int pk = sites[q];
+                    const int x = pk & 1023, y = (pk >> 10) & 1023, z = pk >> 20;
+                    const int si = (x * Y + y) * Z + z;
+                    double hx_, hy_, hz_, gx_ = 0.0, gy_ = 0.0, gz_ = 0.0;
+                    double ax[NPL], ay[NPL], az[NPL], bx[NPL], by[NPL], bz[NPL], la[NPL], lb[NPL];
+#pragma unroll
+                    for (int m = 0; m < NPL; m++) {
+                        // periodic wrap of x +- dx etc. (|d| <= rc < length)
+                        int xa = x + pdx[m], ya = y + pdy[m], za = z + pdz[m];
+                        int xb = x - pdx[m], yb = y - pdy[m], zb = z - pdz[m];
+                        xa -= xa >= X ? X : 0; xa += xa < 0 ? X : 0;
+                        ya -= ya >= Y ? Y : 0; ya += ya < 0 ? Y : 0;
+                        za -= za >= Z ? Z : 0; za += za < 0 ? Z : 0;
+                        xb -= xb >= X ? X : 0; xb += xb < 0 ? X : 0;
+                        yb -= yb >= Y ? Y : 0; yb += yb < 0 ? Y : 0;
+                        zb -= zb >= Z ? Z : 0; zb += zb < 0 ? Z : 0;
+                        const int ia = (xa * Y + ya) * Z + za, ib = (xb * Y + yb) * Z + zb;
+                        ax[m] = sx[ia]; ay[m] = sy[ia]; az[m] = sz[ia];
+                        bx[m] = sx[ib]; by[m] = sy[ib]; bz[m] = sz[ib];
+                        if (MIXED) { la[m] = sl[ia]; lb[m] = sl[ib]; }
+                    }
+                    const int pq = q - p0;
+                    const double px = prop[pq], py = prop[PS + pq], pz = prop[2 * PS + pq];
+                    const double thr = prop[3 * PS + pq];
+                    const double ox = sx[si], oy = sy[si], oz = sz[si];
+                    const double Li = MIXED ? sl[si] : rep.L0;
+#pragma unroll
+                    for (int m = 0; m < NPL; m++) {
+                        const double ux = ax[m] + bx[m], uy = ay[m] + by[m], uz = az[m] + bz[m];
+                        double mx, my, mz;
+                        if (MIXED) {
+                            mx = la[m] * ax[m] + lb[m] * bx[m];
+                            my = la[m] * ay[m] + lb[m] * by[m];
+                            mz = la[m] * az[m] + lb[m] * bz[m];
+                        } else {
+                            mx = ux; my = uy; mz = uz;
+                        }
+                        const double sdot = nx[m] * mx + ny[m] * my + nz[m] * mz;
+                        const double ts = w3[m] * sdot;
+                        if (m == 0) {
+                            hx_ = fma(ts, nx[m], wn0 * mx);
+                            hy_ = fma(ts, ny[m], wn0 * my);
+                            hz_ = fma(ts, nz[m], wn0 * mz);
+                            if (MIXED) { gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz; }
+                        } else {
+                            hx_ = fma(ts, nx[m], fma(w[m], mx, hx_));
+                            hy_ = fma(ts, ny[m], fma(w[m], my, hy_));
+                            hz_ = fma(ts, nz[m], fma(w[m], mz, hz_));
+                        }
+                    }
+                    const double coef = MIXED ? Li : Li * rep.L0;
+                    const double dx = px - ox, dy = py - oy, dz = pz - oz;
+                    double partial = coef * (dx * hx_ + dy * hy_ + dz * hz_);
+                    if (MIXED) partial -= J * (dx * gx_ + dy * gy_ + dz * gz_);
+                    partial = group_sum_dpp<G>(partial);
+                    double dE = partial + (dx * E0 + dy * E1 + dz * E2);
+                    if (K > 0.0) dE += -K * (fabs(px) - fabs(ox) + fabs(py) - fabs(oy));
+                    if (part == 0 && Li != 0.0) {   // vacancies never move
+                        if (dE < thr) {
+                            sx[si] = px; sy[si] = py; sz[si] = pz;
+                            nacc++;
+                        } else nrej++;
+                    }
+                }
+            }
+            __syncthreads();   // the colour is done before the next one reads it
+          }
+            k0 = k1;
+        }
+    }
+
+    // ---- write back and count ----
+    for (int i = tid; i < N; i += BLK) {
+        double4 v;
+        v.x = sx[i]; v.y = sy[i]; v.z = sz[i];
+        v.w = MIXED ? sl[i] : rep.L0;
+        lat[i] = v;
+    }
+    unsigned long long ca = nacc, cr = nrej;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        ca += __shfl_xor(ca, o, 64);
+        cr += __shfl_xor(cr, o, 64);
+    }
+    __shared__ unsigned long long s_cnt[2 * (BLK / 64)];
+    if ((tid & 63) == 0) { s_cnt[2 * (tid >> 6)] = ca; s_cnt[2 * (tid >> 6) + 1] = cr; }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long A_ = 0, R_ = 0;
+        for (int w_ = 0; w_ < BLK / 64; w_++) { A_ += s_cnt[2 * w_]; R_ += s_cnt[2 * w_ + 1]; }
+        t.counters[2 * r] += A_;
+        t.counters[2 * r + 1] += R_;
+    }
+}
+
+}  // namespace sn

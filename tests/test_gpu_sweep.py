@@ -36,6 +36,14 @@ CASES = {
     "25x25x1_gather_512": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=2, block_threads=512)),
     "18x20x22_gather_512_cubic": (dict(X=18, Y=20, Z=22, T=350), False,
                                   dict(kernel=2, block_threads=512, fine_colouring=1)),
+    # whole-lattice sweeps (kernel 3): one workgroup per replica, periodic colouring
+    "25x25x1_torus": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=3)),
+    "16cube_torus_mixed_field_K": (dict(X=16, Y=16, Z=16, T=250, Efield=(0.05, 0.0, 0.1), K=0.3), True,
+                                   dict(kernel=3)),
+    "12cube_rc2_torus": (dict(X=12, Y=12, Z=12, DipoleCutOff=2, T=300), True, dict(kernel=3)),
+    "16cube_rc4_torus": (dict(X=16, Y=16, Z=16, DipoleCutOff=4, T=300), True, dict(kernel=3)),
+    "32x24x1_torus_mixed_K": (dict(X=32, Y=24, Z=1, T=200, K=0.2), True, dict(kernel=3)),
+    "16cube_torus_constrain_x": (dict(X=16, Y=16, Z=16, T=300, ConstrainToX=True), False, dict(kernel=3)),
 }
 
 
@@ -228,3 +236,18 @@ def test_multi_sweep_graph_matches_single_sweeps(sn):
             out.append((ctx.get_lattice(), ctx.counters()))
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1] and out[0][1]["attempted"] == 37 * rp.N
+
+
+def test_torus_batch_and_limits(sn, oracle):
+    """Whole-lattice sweeps of a batch of small replicas (one workgroup each,
+    one periodic colouring for all), against the oracle; sizes with no periodic
+    colouring, or too large for LDS, are refused at creation."""
+    reps = [sn.ReplicaParams(L, L, L, T=250 + 20 * i, seed=70 + i) for i, L in enumerate((16, 12, 8))]
+    print_case("torus_batch", sizes=(16, 12, 8))
+    rng = np.random.default_rng(12)
+    lats = [rand_lattice(rp.N, rng, mixed=True) for rp in reps]
+    run_pair(sn, oracle, reps, lats, dict(kernel=3), nsweeps=3)
+    for bad in (dict(X=17, Y=19, Z=21), dict(X=24, Y=24, Z=24)):
+        with pytest.raises(sn.SnError) as e:
+            sn.StarryNight(sn.ReplicaParams(T=300, **bad), sn.TuningParams(kernel=3))
+        assert e.value.status == 5
