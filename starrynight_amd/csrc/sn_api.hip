@@ -304,7 +304,8 @@ static int setup_torus(sn_ctx *c)
     // colour has enough sites to keep them busy
     const int np = c->npairs;
     int G, NPL, blk = 256;
-    if (np <= 16) { G = 4; NPL = 4; }
+    if (np <= 16 && c->torus_mc * 16 <= 512) { G = 16; NPL = 1; blk = c->torus_mc * 16 > 256 ? 512 : 256; }
+    else if (np <= 16) { G = 4; NPL = 4; }
     else if (np <= 32) { G = 8; NPL = 4; }
     else if (np <= 64) {
         if (c->torus_mc * 16 >= 1024) { G = 32; NPL = 2; blk = 1024; }
@@ -312,6 +313,15 @@ static int setup_torus(sn_ctx *c)
     }
     else if (np <= 128) { G = 64; NPL = 2; blk = 1024; }
     else return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: DipoleCutOff too large");
+    if (c->tuning.lanes_per_site > 0) {
+        // requested lanes per site: pairs per lane to cover the table, threads
+        // for up to one step per colour
+        G = c->tuning.lanes_per_site;
+        NPL = 1;
+        while (NPL * G < np) NPL <<= 1;
+        blk = c->torus_mc * G > 512 ? 1024 : c->torus_mc * G > 256 ? 512 : 256;
+        if (!pick_torus(G, NPL, false, blk) && blk < 1024 && pick_torus(G, NPL, false, 1024)) blk = 1024;
+    }
     c->torus_G = G; c->torus_NPL = NPL; c->torus_blk = blk;
     // proposal slots: whole runs of colours per draw, as many as LDS leaves room
     // for (at most the largest replica, at least the largest colour)

@@ -160,10 +160,11 @@ extern "C" int sn_autotune(const sn_params *params, int32_t n_replicas, int32_t 
         tiles = {{8, 8, 8}, {8, 8, 6}, {8, 8, 10}, {6, 8, 8}, {6, 6, 6}, {8, 6, 6}, {4, 8, 8}, {8, 8, 4}, {4, 4, 8}};
     std::vector<int> colourings = (!two_d && rc == 3) ? std::vector<int>{3, 2, 1} : std::vector<int>{2, 1};
     std::vector<sn_tuning> cands;
-    {
+    for (int lanes : {0, 8, 16}) {
         // whole-lattice sweeps (small replicas only; refused at creation otherwise)
         sn_tuning t{};
         t.kernel = 3;
+        t.lanes_per_site = lanes;
         cands.push_back(t);
     }
     // (kernel, threads per tile): gather with 8 or 4 waves, field-update.  The

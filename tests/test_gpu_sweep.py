@@ -44,6 +44,9 @@ CASES = {
     "16cube_rc4_torus": (dict(X=16, Y=16, Z=16, DipoleCutOff=4, T=300), True, dict(kernel=3)),
     "32x24x1_torus_mixed_K": (dict(X=32, Y=24, Z=1, T=200, K=0.2), True, dict(kernel=3)),
     "16cube_torus_constrain_x": (dict(X=16, Y=16, Z=16, T=300, ConstrainToX=True), False, dict(kernel=3)),
+    "25x25x1_torus_lanes16": (dict(X=25, Y=25, Z=1, T=300, Efield=(0.0, 0.02, 0.0)), True,
+                              dict(kernel=3, lanes_per_site=16)),
+    "16cube_torus_lanes32": (dict(X=16, Y=16, Z=16, T=300), True, dict(kernel=3, lanes_per_site=32)),
 }
 
 
