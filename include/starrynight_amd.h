@@ -72,7 +72,8 @@ typedef struct sn_tuning {
  * Replaces the reference's load_config() + malloc of lattice + gen_neighbour()
  * (src/starrynight-main.c:137-180, src/starrynight-montecarlo-core.c:38-72).
  * tuning may be NULL.  Lattices start zeroed; load them with sn_set_lattice or
- * sn_init_lattice_random. */
+ * sn_init_lattice_random.  On failure *out may still hold the partial context
+ * (for sn_last_error); release it with sn_destroy. */
 int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuning, sn_ctx **out);
 int sn_destroy(sn_ctx *ctx);
 const char *sn_last_error(const sn_ctx *ctx);
