@@ -122,6 +122,27 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     }
     for (int i = tid; i < s_gchunk_total(sh) * SW; i += BLK) s_gstep[i] = a.gsteps[sh.gstep_off + i];
 
+    // proposal and Metropolis threshold of the chunk's site k (region index ri)
+    const unsigned sweep = a.sweep_ctr[tile.replica];
+    const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
+    auto draw = [&](int k, int ri) __attribute__((always_inline)) {
+        const int iz = ri % RZ, q = ri / RZ;
+        const int iy = q % RY, ix = q / RY;
+        const unsigned gsite = (unsigned)(tile_base + ((long long)(ix - hx) * rep.Y + (iy - hy)) * rep.Z + (iz - hz));
+        double px, py, pz, u;
+        counter_draw(rep.seed, (uint32_t)tile.replica, sweep, gsite, rep.constrain_to_x, px, py, pz, u);
+        prop[k] = px;
+        prop[PS + k] = py;
+        prop[2 * PS + k] = pz;
+        prop[3 * PS + k] = metropolis_threshold(u, rep.beta);
+    };
+    // the first chunk's proposals are drawn while the staging loads are in
+    // flight (its site list is read from global memory: the LDS copy is not
+    // ready before the barrier)
+    const int n0 = a.class_start[sh.class_off + a.chunk_col[sh.chunk_off + 1]];
+    const int ri0 = tid < n0 ? a.class_ri[sh.sites_off + tid] : 0;
+    bool drawn = false;
+
     // ---- stage tile + halo (periodic wrap), 4 loads in flight per lane ----
     const double4 *lat = a.lat + rep.base;
     {
@@ -144,6 +165,14 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                 if (iz >= RZ) { iz -= RZ; iy++; }
                 if (iy >= RY) { iy -= RY; ix++; }
             }
+            if (!drawn) {
+                // after this batch's loads have issued (the fence keeps them
+                // above it), before their data is needed
+                drawn = true;
+                int r0 = ri0;
+                asm volatile("" : "+v"(r0) : : "memory");
+                if (tid < n0) draw(tid, r0);
+            }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (idx[k] < region) {
@@ -153,6 +182,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
             }
         }
     }
+    if (!drawn && tid < n0) draw(tid, ri0);   // lanes past a small region's end
 
     // ---- this lane's share of the neighbour pairs, held in registers ----
     const int part = tid % G;
@@ -173,33 +203,21 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     // nearest-neighbour pairs come first in the table, so they sit in slot 0
     const double nnf = (part < a.npairs && a.pairs[part].nn) ? 1.0 : 0.0;
 
-    const unsigned sweep = a.sweep_ctr[tile.replica];
     const double J = rep.J, K = rep.K, E0 = rep.E[0], E1 = rep.E[1], E2 = rep.E[2];
     // Uniform lengths: the cage-strain sum rides on the nearest-neighbour
     // pair's weight, h' = h - J / L0^2 g, so the partial is L0^2 dp.h' and no
     // separate g is accumulated.  Mixed lengths keep g (L_j sits inside h).
     const double wn0 = (!MIXED && nnf != 0.0 && rep.L0 != 0.0) ? w[0] - J / (rep.L0 * rep.L0) : w[0];
-    const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
     unsigned nacc = 0, nrej = 0;
     __syncthreads();
 
     for (int ch = 0; ch < sh.nchunks; ch++) {
         const int c0 = s_chunk[ch], c1 = s_chunk[ch + 1];
         const int s0 = s_cls[c0], s1 = s_cls[c1];
-        // ---- proposals of this chunk's sites: one site per lane ----
-        if (tid < s1 - s0) {
-            int ri = s_ri[s0 + tid];
-            int iz = ri % RZ, q = ri / RZ;
-            int iy = q % RY, ix = q / RY;
-            unsigned gsite = (unsigned)(tile_base + ((long long)(ix - hx) * rep.Y + (iy - hy)) * rep.Z + (iz - hz));
-            double px, py, pz, u;
-            counter_draw(rep.seed, (uint32_t)tile.replica, sweep, gsite, rep.constrain_to_x, px, py, pz, u);
-            prop[tid] = px;
-            prop[PS + tid] = py;
-            prop[2 * PS + tid] = pz;
-            prop[3 * PS + tid] = metropolis_threshold(u, rep.beta);
-        }
-        __syncthreads();
+        // ---- proposals of this chunk's sites: one site per lane (the first
+        // chunk's were drawn during staging) ----
+        if (ch > 0 && tid < s1 - s0) draw(tid, s_ri[s0 + tid]);
+        if (ch > 0) __syncthreads();
         // this chunk's steps: every group decides one site per step; a step
         // holds sites of one colour and a colour's last step ends with the
         // barrier.  The next step's entry and flag load with this step's data.
