@@ -122,6 +122,26 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     }
     for (int i = tid; i < s_gchunk_total(sh) * SW; i += BLK) s_gstep[i] = a.gsteps[sh.gstep_off + i];
 
+    // ---- this lane's share of the neighbour pairs, held in registers (loaded
+    // first: their latency overlaps the staging) ----
+    const int part = tid % G;
+    const int group = tid / G;
+    int dl[NPL];
+    double nx[NPL], ny[NPL], nz[NPL], w[NPL], w3[NPL];
+#pragma unroll
+    for (int m = 0; m < NPL; m++) {
+        int j = part + m * G;
+        if (j < a.npairs) {
+            PairDev p = a.pairs[j];
+            dl[m] = (p.dx * RY + p.dy) * RZ + p.dz;
+            nx[m] = p.nx; ny[m] = p.ny; nz[m] = p.nz; w[m] = p.w; w3[m] = -3.0 * p.w;
+        } else {
+            dl[m] = 0; nx[m] = 0.0; ny[m] = 0.0; nz[m] = 0.0; w[m] = 0.0; w3[m] = 0.0;
+        }
+    }
+    // nearest-neighbour pairs come first in the table, so they sit in slot 0
+    const double nnf = (part < a.npairs && a.pairs[part].nn) ? 1.0 : 0.0;
+
     // proposal and Metropolis threshold of the chunk's site k (region index ri)
     const unsigned sweep = a.sweep_ctr[tile.replica];
     const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
@@ -183,25 +203,6 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         }
     }
     if (!drawn && tid < n0) draw(tid, ri0);   // lanes past a small region's end
-
-    // ---- this lane's share of the neighbour pairs, held in registers ----
-    const int part = tid % G;
-    const int group = tid / G;
-    int dl[NPL];
-    double nx[NPL], ny[NPL], nz[NPL], w[NPL], w3[NPL];
-#pragma unroll
-    for (int m = 0; m < NPL; m++) {
-        int j = part + m * G;
-        if (j < a.npairs) {
-            PairDev p = a.pairs[j];
-            dl[m] = (p.dx * RY + p.dy) * RZ + p.dz;
-            nx[m] = p.nx; ny[m] = p.ny; nz[m] = p.nz; w[m] = p.w; w3[m] = -3.0 * p.w;
-        } else {
-            dl[m] = 0; nx[m] = 0.0; ny[m] = 0.0; nz[m] = 0.0; w[m] = 0.0; w3[m] = 0.0;
-        }
-    }
-    // nearest-neighbour pairs come first in the table, so they sit in slot 0
-    const double nnf = (part < a.npairs && a.pairs[part].nn) ? 1.0 : 0.0;
 
     const double J = rep.J, K = rep.K, E0 = rep.E[0], E1 = rep.E[1], E2 = rep.E[2];
     // Uniform lengths: the cage-strain sum rides on the nearest-neighbour
