@@ -14,9 +14,14 @@ typedef void (*FieldFn)(FieldArgs, int, int);
 // 16 lanes share the scatter of one accepted site in the field-update kernel.
 constexpr int kFieldG = 16;
 
+// Staged-region size with compile-time-stride gather instances: the 14^3
+// region of an 8^3 tile at DipoleCutOff 3 (10^3 at 2, 6^3 at 4, 12^3 at 1).
+constexpr int kFixedRegion = 14 * 14 * 14;
+
 // per-phase gather kernel for G lanes per site, NPL pairs per lane and blk
-// threads per workgroup (or null)
-SweepFn pick_sweep(int G, int NPL, bool mixed, int blk);
+// threads per workgroup (or null); max_region == kFixedRegion selects the
+// compile-time-stride instance where one is built
+SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int max_region = 0);
 typedef void (*TorusFn)(TorusArgs, int);
 // whole-lattice sweep kernel of small replicas (or null)
 TorusFn pick_torus(int G, int NPL, bool mixed, int blk);

@@ -5,8 +5,14 @@
 
 namespace sn {
 
-SweepFn pick_sweep(int G, int NPL, bool mixed, int blk)
+SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int max_region)
 {
+#define SN_FIXED(g, n, b)                                                                            \
+    if (max_region == kFixedRegion && G == g && NPL == n && blk == b)                                \
+        return mixed ? (SweepFn)tile_sweep_kernel<g, n, true, b, kFixedRegion>                        \
+                     : (SweepFn)tile_sweep_kernel<g, n, false, b, kFixedRegion>;
+    SN_FIXED(16, 4, 256) SN_FIXED(16, 2, 256) SN_FIXED(8, 8, 256)
+#undef SN_FIXED
 #define SN_CASE(g, n, b)                                                                        \
     if (G == g && NPL == n && blk == b) return mixed ? (SweepFn)tile_sweep_kernel<g, n, true, b>  \
                                                      : (SweepFn)tile_sweep_kernel<g, n, false, b>;

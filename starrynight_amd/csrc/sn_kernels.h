@@ -81,7 +81,14 @@ __device__ __forceinline__ int s_gchunk_total(const ShapeDev &sh) { return sh.gs
 // separate instructions: a read2 of x and y at a stride that is a multiple of
 // the bank cycle puts both halves in one bank (measured: 2.9 conflict cycles
 // per LDS instruction against 1.3, and 9% slower at 8^3 tiles).
-template <int G, int NPL, bool MIXED, int BLK>
+// MRC > 0 fixes the stride of the staged arrays at compile time (the launch's
+// max_region must equal it): the y and z reads then take immediate LDS offsets
+// from the x address instead of an address add each (measured 2% at 8^3 tiles).
+// -3 w n of a lane's pairs (present only where the kernel keeps it)
+template <int N, bool ON> struct PairTerm { double x[N], y[N], z[N]; };
+template <int N> struct PairTerm<N, false> {};
+
+template <int G, int NPL, bool MIXED, int BLK, int MRC = 0>
 __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_tile, int ntiles)
 {
     extern __shared__ double smem[];
@@ -98,7 +105,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     const int RX = sh.tx + 2 * hx;
     const int region = RX * RY * RZ;
 
-    const int MR = a.max_region;                           // stride of the staged arrays
+    const int MR = MRC > 0 ? MRC : a.max_region;          // stride of the staged arrays
     double *sx = smem;
     double *sy = sx + MR;
     double *sz = sy + MR;
@@ -138,6 +145,16 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         } else {
             dl[m] = 0; nx[m] = 0.0; ny[m] = 0.0; nz[m] = 0.0; w[m] = 0.0; w3[m] = 0.0;
         }
+    }
+    // TN: t = -3 w n per pair in registers, one multiply fewer per pair for
+    // three more registers.  Only the fixed-stride instances take it: their
+    // LDS (>= 66 KB) allows two workgroups per CU, so up to 256 VGPRs cost no
+    // occupancy, while smaller tiles run three workgroups per CU (<= 168).
+    constexpr bool TN = MRC > 0;
+    PairTerm<NPL, TN> t3;
+    if constexpr (TN) {
+#pragma unroll
+        for (int m = 0; m < NPL; m++) { t3.x[m] = w3[m] * nx[m]; t3.y[m] = w3[m] * ny[m]; t3.z[m] = w3[m] * nz[m]; }
     }
     // nearest-neighbour pairs come first in the table, so they sit in slot 0
     const double nnf = (part < a.npairs && a.pairs[part].nn) ? 1.0 : 0.0;
@@ -266,9 +283,22 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                         mx = ux; my = uy; mz = uz;
                     }
                     const double s = nx[m] * mx + ny[m] * my + nz[m] * mz;
-                    const double ts = w3[m] * s;
                     // the first pair starts the sums (no adds of +0.0); each
                     // component is two dependent FMAs
+                    if constexpr (TN) {
+                        if (m == 0) {
+                            hx_ = fma(s, t3.x[m], wn0 * mx);
+                            hy_ = fma(s, t3.y[m], wn0 * my);
+                            hz_ = fma(s, t3.z[m], wn0 * mz);
+                            if (MIXED) { gx_ = nnf * ux; gy_ = nnf * uy; gz_ = nnf * uz; }
+                        } else {
+                            hx_ = fma(s, t3.x[m], fma(w[m], mx, hx_));
+                            hy_ = fma(s, t3.y[m], fma(w[m], my, hy_));
+                            hz_ = fma(s, t3.z[m], fma(w[m], mz, hz_));
+                        }
+                        continue;
+                    }
+                    const double ts = w3[m] * s;
                     if (m == 0) {
                         hx_ = fma(ts, nx[m], wn0 * mx);
                         hy_ = fma(ts, ny[m], wn0 * my);

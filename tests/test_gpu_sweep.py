@@ -36,6 +36,13 @@ CASES = {
     "25x25x1_gather_512": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=2, block_threads=512)),
     "18x20x22_gather_512_cubic": (dict(X=18, Y=20, Z=22, T=350), False,
                                   dict(kernel=2, block_threads=512, fine_colouring=1)),
+    # staged region of exactly 14^3 sites: the compile-time-stride gather instances
+    "16cube_gather_region14_uniform": (dict(X=16, Y=16, Z=16, T=300, K=0.1, Efield=(0.0, 0.0, 0.05)), False,
+                                       dict(kernel=2, tile=(8, 8, 8))),
+    "16cube_gather_region14_lanes8": (dict(X=16, Y=16, Z=16, T=300), True,
+                                      dict(kernel=2, tile=(8, 8, 8), lanes_per_site=8)),
+    "20cube_rc2_gather_region14": (dict(X=20, Y=20, Z=20, DipoleCutOff=2, T=300), True,
+                                   dict(kernel=2, tile=(10, 10, 10), lanes_per_site=16)),
     # whole-lattice sweeps (kernel 3): one workgroup per replica, periodic colouring
     "25x25x1_torus": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=3)),
     "16cube_torus_mixed_field_K": (dict(X=16, Y=16, Z=16, T=250, Efield=(0.05, 0.0, 0.1), K=0.3), True,
