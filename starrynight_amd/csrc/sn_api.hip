@@ -587,8 +587,9 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         c->energy_tiled = true;
     }
     for (int m = 0; m < 2; m++) {
-        const void *f = c->use_field ? (const void *)pick_field(c->rc, c->two_d, m == 1)
-                                     : (const void *)pick_sweep(G, NPL, m == 1, c->blk, c->sched.max_region);
+        const void *f = c->use_field
+                            ? (const void *)pick_field(c->rc, c->two_d, m == 1)
+                            : (const void *)pick_sweep(G, NPL, m == 1, c->blk, c->sched.max_region, c->prop_chunk);
         HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need_lds));
     }
     return SN_OK;
@@ -824,7 +825,7 @@ static int launch_coarse(sn_ctx *c, bool mixed, int first, int n)
         hipLaunchKernelGGL(pick_field(c->rc, c->two_d, mixed), dim3(n), dim3(kBlock), field_lds(c, mixed),
                            c->stream, fa, first, n);
     } else {
-        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed, c->blk, c->sched.max_region), dim3(n), dim3(c->blk),
+        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed, c->blk, c->sched.max_region, c->prop_chunk), dim3(n), dim3(c->blk),
                            sweep_lds(c, mixed),
                            c->stream, sweep_args(c), first, n);
     }

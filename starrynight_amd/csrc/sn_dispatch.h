@@ -19,9 +19,9 @@ constexpr int kFieldG = 16;
 constexpr int kFixedRegion = 14 * 14 * 14;
 
 // per-phase gather kernel for G lanes per site, NPL pairs per lane and blk
-// threads per workgroup (or null); max_region == kFixedRegion selects the
-// compile-time-stride instance where one is built
-SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int max_region = 0);
+// threads per workgroup (or null); max_region == kFixedRegion with proposal
+// chunks of kBlock selects the compile-time-stride instance where one is built
+SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int max_region = 0, int prop_chunk = 0);
 typedef void (*TorusFn)(TorusArgs, int);
 // whole-lattice sweep kernel of small replicas (or null)
 TorusFn pick_torus(int G, int NPL, bool mixed, int blk);

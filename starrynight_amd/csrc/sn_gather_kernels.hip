@@ -5,10 +5,10 @@
 
 namespace sn {
 
-SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int max_region)
+SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int max_region, int prop_chunk)
 {
 #define SN_FIXED(g, n, b)                                                                            \
-    if (max_region == kFixedRegion && G == g && NPL == n && blk == b)                                \
+    if (max_region == kFixedRegion && prop_chunk == kBlock && G == g && NPL == n && blk == b)       \
         return mixed ? (SweepFn)tile_sweep_kernel<g, n, true, b, kFixedRegion>                        \
                      : (SweepFn)tile_sweep_kernel<g, n, false, b, kFixedRegion>;
     SN_FIXED(16, 4, 256) SN_FIXED(16, 2, 256) SN_FIXED(8, 8, 256)

@@ -82,8 +82,9 @@ __device__ __forceinline__ int s_gchunk_total(const ShapeDev &sh) { return sh.gs
 // the bank cycle puts both halves in one bank (measured: 2.9 conflict cycles
 // per LDS instruction against 1.3, and 9% slower at 8^3 tiles).
 // MRC > 0 fixes the stride of the staged arrays at compile time (the launch's
-// max_region must equal it): the y and z reads then take immediate LDS offsets
-// from the x address instead of an address add each (measured 2% at 8^3 tiles).
+// max_region must equal it, and its prop_stride kBlock): the y and z reads then
+// take immediate LDS offsets from the x address instead of an address add each
+// (measured 2% at 8^3 tiles), and so do the proposal reads.
 // -3 w n of a lane's pairs (present only where the kernel keeps it)
 template <int N, bool ON> struct PairTerm { double x[N], y[N], z[N]; };
 template <int N> struct PairTerm<N, false> {};
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     double *sz = sy + MR;
     double *sl = sz + MR;                                  // lengths (MIXED only)
     double *prop = MIXED ? sl + MR : sl;                   // 4 x prop_stride: px, py, pz, threshold
-    const int PS = a.prop_stride;
+    const int PS = MRC > 0 ? kBlock : a.prop_stride;        // (fixed-stride launches draw chunks of kBlock)
     int *s_ri = (int *)(prop + 4 * PS);                    // region index of class sites
     int *s_cls = s_ri + a.max_tile_sites;                  // class starts, then chunk colours
     int *s_chunk = s_cls + a.max_cls + 1;
