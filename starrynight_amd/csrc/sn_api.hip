@@ -76,6 +76,8 @@ struct sn_ctx {
     // whole-lattice sweeps of small replicas (kernel 3): one workgroup per replica
     bool torus = false;
     Sublattice torus_lat{};
+    std::vector<std::vector<int32_t>> torus_order, torus_cstart;   // per replica: sites by colour, colour starts
+    int torus_ncol = 0;                                             // colours per sweep (shorter colourings padded)
     int torus_G = 0, torus_NPL = 0, torus_blk = 0, torus_ms = 0, torus_mc = 0, torus_pc = 0;
     int *d_torus_sites = nullptr, *d_torus_off = nullptr, *d_torus_cs = nullptr;
     unsigned long long *d_rep_counters = nullptr;   // 2 per replica (torus sweeps)
@@ -278,41 +280,98 @@ static size_t torus_lds(const sn_ctx *c, bool mixed)
            (size_t)c->torus_ms * sizeof(int);
 }
 
-static int setup_torus(sn_ctx *c)
+// lanes per site, pairs per lane and threads for npairs pairs and colours of
+// at most mc sites: 1024 threads when a colour has enough sites to keep them
+// busy (16 lanes x up to 64 sites for the greedy colourings, wide16)
+static bool torus_shape(int np, int mc, bool wide16, int &G, int &NPL, int &blk)
 {
-    if (!periodic_sublattice(c->geom, c->rc, kTorusMaxIndex, c->torus_lat))
-        return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: no periodic colouring of these lattice sizes");
-    std::vector<int> sites, off, cs;
-    int mc = 0;
-    for (int r = 0; r < c->R; r++) {
-        std::vector<int32_t> s, st_;
-        torus_classes(c->torus_lat, c->geom[r], s, st_);
-        off.push_back((int)sites.size());
-        const ReplicaGeom &g = c->geom[r];
-        if (g.X > 1023 || g.Y > 1023 || g.Z > 1023)
-            return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: an axis is longer than 1023");
-        for (int32_t si : s) {   // packed coordinates x | y << 10 | z << 20
-            const int z = si % g.Z, xy = si / g.Z, y = xy % g.Y, x = xy / g.Y;
-            sites.push_back(x | (y << 10) | (z << 20));
-        }
-        cs.insert(cs.end(), st_.begin(), st_.end());
-        for (size_t k = 0; k + 1 < st_.size(); k++) mc = std::max(mc, st_[k + 1] - st_[k]);
-        c->torus_ms = std::max<int>(c->torus_ms, (int)c->reps[r].N);
+    blk = 256;
+    if (np <= 16 && mc * 16 <= (wide16 ? 1024 : 512)) {
+        G = 16; NPL = 1; blk = mc * 16 > 512 ? 1024 : mc * 16 > 256 ? 512 : 256;
     }
-    c->torus_mc = std::max(mc, 1);
-    // lanes per site and pairs per lane for the pair count; 1024 threads when a
-    // colour has enough sites to keep them busy
-    const int np = c->npairs;
-    int G, NPL, blk = 256;
-    if (np <= 16 && c->torus_mc * 16 <= 512) { G = 16; NPL = 1; blk = c->torus_mc * 16 > 256 ? 512 : 256; }
     else if (np <= 16) { G = 4; NPL = 4; }
     else if (np <= 32) { G = 8; NPL = 4; }
     else if (np <= 64) {
-        if (c->torus_mc * 16 >= 1024) { G = 32; NPL = 2; blk = 1024; }
+        if (mc * 16 >= 1024) { G = 32; NPL = 2; blk = 1024; }
         else { G = 16; NPL = 4; }
     }
     else if (np <= 128) { G = 64; NPL = 2; blk = 1024; }
-    else return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: DipoleCutOff too large");
+    else return false;
+    return true;
+}
+
+// Steps plus barriers of one sweep: each colour takes ceil(size / groups)
+// steps and one barrier.
+static long long torus_cost(const std::vector<std::vector<int32_t>> &cstart, int groups)
+{
+    long long cost = 0;
+    for (const auto &st : cstart)
+        for (size_t k = 0; k + 1 < st.size(); k++) cost += (st[k + 1] - st[k] + groups - 1) / groups + 1;
+    return cost;
+}
+
+static int setup_torus(sn_ctx *c)
+{
+    const int np = c->npairs;
+    for (const ReplicaGeom &g : c->geom)
+        if (g.X > 1023 || g.Y > 1023 || g.Z > 1023)
+            return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: an axis is longer than 1023");
+    for (int r = 0; r < c->R; r++) c->torus_ms = std::max<int>(c->torus_ms, (int)c->reps[r].N);
+    // the x, y, z, length arrays and the site table alone (before colouring)
+    if ((size_t)c->torus_ms * (4 * sizeof(double) + sizeof(int)) > 150 * 1024)
+        return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: a replica does not fit in LDS (" +
+                                               std::to_string(c->torus_ms) + " sites)");
+    // Two colourings: a sublattice holding every replica box (the same colours
+    // for all replicas), and, at up to 16 pairs, a greedy colouring per replica
+    // with colours of at most 64 sites (one step of 16 lanes x 64 sites).
+    // The one with fewer steps plus barriers per sweep is kept.
+    auto max_class = [](const std::vector<std::vector<int32_t>> &cst) {
+        int mc = 1;
+        for (const auto &st : cst)
+            for (size_t k = 0; k + 1 < st.size(); k++) mc = std::max(mc, st[k + 1] - st[k]);
+        return mc;
+    };
+    std::vector<std::vector<int32_t>> lo, ls, go, gs;
+    long long lcost = -1, gcost = -1;
+    if (periodic_sublattice(c->geom, c->rc, kTorusMaxIndex, c->torus_lat)) {
+        lo.resize(c->R); ls.resize(c->R);
+        for (int r = 0; r < c->R; r++) torus_classes(c->torus_lat, c->geom[r], lo[r], ls[r]);
+        int G, NPL, blk;
+        if (torus_shape(np, max_class(ls), false, G, NPL, blk)) lcost = torus_cost(ls, blk / G);
+    }
+    if (np <= 16) {
+        go.resize(c->R); gs.resize(c->R);
+        bool ok = true;
+        for (int r = 0; r < c->R && ok; r++) {
+            greedy_torus_classes(c->geom[r], c->rc, 64, go[r], gs[r]);
+            ok = valid_torus_classes(c->geom[r], c->rc, go[r], gs[r]);
+        }
+        int G, NPL, blk;
+        if (ok && torus_shape(np, max_class(gs), true, G, NPL, blk)) gcost = torus_cost(gs, blk / G);
+    }
+    if (lcost < 0 && gcost < 0)
+        return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: no periodic colouring of these lattice sizes");
+    const bool greedy = gcost >= 0 && (lcost < 0 || gcost < lcost);
+    c->torus_order = greedy ? go : lo;
+    c->torus_cstart = greedy ? gs : ls;
+    c->torus_ncol = 0;
+    for (const auto &st : c->torus_cstart) c->torus_ncol = std::max(c->torus_ncol, (int)st.size() - 1);
+    std::vector<int> sites, off, cs;
+    for (int r = 0; r < c->R; r++) {
+        const ReplicaGeom &g = c->geom[r];
+        off.push_back((int)sites.size());
+        for (int32_t si : c->torus_order[r]) {   // packed coordinates x | y << 10 | z << 20
+            const int z = si % g.Z, xy = si / g.Z, y = xy % g.Y, x = xy / g.Y;
+            sites.push_back(x | (y << 10) | (z << 20));
+        }
+        std::vector<int32_t> &st_ = c->torus_cstart[r];
+        while ((int)st_.size() < c->torus_ncol + 1) st_.push_back(st_.back());   // empty colours
+        cs.insert(cs.end(), st_.begin(), st_.end());
+    }
+    c->torus_mc = max_class(c->torus_cstart);
+    int G, NPL, blk;
+    if (!torus_shape(np, c->torus_mc, greedy, G, NPL, blk))
+        return fail(c, SN_ERR_UNSUPPORTED, "whole-lattice sweep: DipoleCutOff too large");
     if (c->tuning.lanes_per_site > 0) {
         // requested lanes per site: pairs per lane to cover the table, threads
         // for up to one step per colour
@@ -844,7 +903,7 @@ static int launch_torus(sn_ctx *c, int nsw, std::vector<hipEvent_t> *events)
     TorusArgs t;
     t.lat = c->d_lat; t.reps = c->d_reps; t.pairs = c->d_pairs; t.npairs = c->npairs;
     t.sites = c->d_torus_sites; t.site_off = c->d_torus_off; t.col_start = c->d_torus_cs;
-    t.ncol = c->torus_lat.colours(); t.sweep_ctr = c->d_sweep; t.counters = c->d_rep_counters;
+    t.ncol = c->torus_ncol; t.sweep_ctr = c->d_sweep; t.counters = c->d_rep_counters;
     t.max_sites = c->torus_ms; t.prop_cap = c->torus_pc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (events) {
@@ -1377,7 +1436,7 @@ int sn_schedule_info(sn_ctx *c, int32_t r, int64_t *n_sites, int32_t *n_phases)
     if (st) return st;
     if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     std::vector<int32_t> order, ps;
-    if (c->torus) torus_classes(c->torus_lat, c->geom[r], order, ps);
+    if (c->torus) { order = c->torus_order[r]; ps = c->torus_cstart[r]; }
     else sweep_order(c->sched, c->geom[r], r, order, ps);
     if (n_sites) *n_sites = (int64_t)order.size();
     if (n_phases) *n_phases = (int32_t)ps.size() - 1;
@@ -1390,7 +1449,7 @@ int sn_schedule_order(sn_ctx *c, int32_t r, int32_t *order, int32_t *phase_start
     if (st) return st;
     if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
     std::vector<int32_t> o, ps;
-    if (c->torus) torus_classes(c->torus_lat, c->geom[r], o, ps);
+    if (c->torus) { o = c->torus_order[r]; ps = c->torus_cstart[r]; }
     else sweep_order(c->sched, c->geom[r], r, o, ps);
     if (order) std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
     if (phase_start) std::memcpy(phase_start, ps.data(), ps.size() * sizeof(int32_t));
@@ -1412,7 +1471,7 @@ int sn_sweep_config(sn_ctx *c, int32_t *tiles, int32_t *launches, int32_t *lanes
         if (c->sched.coarse_start[cc + 1] > c->sched.coarse_start[cc]) nl++;
     int maxcol = 0;
     for (const TileShape &s : c->sched.shapes) maxcol = std::max(maxcol, s.ncol);
-    if (c->torus) { nl = 1; maxcol = c->torus_lat.colours(); }
+    if (c->torus) { nl = 1; maxcol = c->torus_ncol; }
     if (tiles) *tiles = c->torus ? 0 : (int32_t)c->sched.tiles.size();
     if (launches) *launches = nl;
     if (lanes) *lanes = c->torus ? c->torus_G : c->G;

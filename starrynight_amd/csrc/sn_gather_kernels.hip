@@ -33,6 +33,7 @@ TorusFn pick_torus(int G, int NPL, bool mixed, int blk)
                                                      : (TorusFn)torus_sweep_kernel<g, n, false, b>;
     SN_CASE(4, 4, 256) SN_CASE(8, 4, 256) SN_CASE(16, 4, 256) SN_CASE(32, 2, 1024) SN_CASE(64, 2, 1024)
     SN_CASE(8, 2, 256) SN_CASE(16, 1, 256) SN_CASE(16, 1, 512) SN_CASE(16, 2, 512) SN_CASE(32, 1, 512)
+    SN_CASE(16, 1, 1024) SN_CASE(8, 2, 512)
 #undef SN_CASE
     return nullptr;
 }

@@ -316,4 +316,104 @@ void torus_classes(const Sublattice &lat, const ReplicaGeom &g, std::vector<int3
     }
 }
 
+// neighbour sites of every site (periodic wrap), as linear indices
+static std::vector<std::vector<int32_t>> torus_neighbours(const ReplicaGeom &g, int rc)
+{
+    const std::vector<Offset> offs = neighbour_offsets(rc, g.Z == 1);
+    const int N = g.X * g.Y * g.Z;
+    std::vector<std::vector<int32_t>> nb(N);
+    for (int x = 0; x < g.X; x++)
+        for (int y = 0; y < g.Y; y++)
+            for (int z = 0; z < g.Z; z++) {
+                std::vector<int32_t> &v = nb[(x * g.Y + y) * g.Z + z];
+                for (const Offset &o : offs) {
+                    const int a = ((x + o.dx) % g.X + g.X) % g.X, b = ((y + o.dy) % g.Y + g.Y) % g.Y;
+                    const int c = ((z + o.dz) % g.Z + g.Z) % g.Z;
+                    v.push_back((a * g.Y + b) * g.Z + c);
+                }
+            }
+    return nb;
+}
+
+int greedy_torus_classes(const ReplicaGeom &g, int rc, int max_class, std::vector<int32_t> &sites,
+                         std::vector<int32_t> &start)
+{
+    const int N = g.X * g.Y * g.Z;
+    const std::vector<std::vector<int32_t>> nb = torus_neighbours(g, rc);
+    std::vector<int> col(N), size;
+    std::vector<int> mark;   // mark[c] == s + 1: colour c is taken next to site s
+    auto greedy = [&](const std::vector<int32_t> &order) {
+        std::fill(col.begin(), col.end(), -1);
+        size.clear();
+        for (int32_t s : order) {
+            for (int32_t t : nb[s])
+                if (col[t] >= 0) mark[col[t]] = s + 1;
+            int c = 0;
+            while (c < (int)size.size() && (mark[c] == s + 1 || size[c] >= max_class)) c++;
+            if (c == (int)size.size()) size.push_back(0);
+            col[s] = c;
+            size[c]++;
+        }
+    };
+    mark.assign(N + 1, 0);
+    // first pass: a sheared raster, (x + 4 y) mod X, then index
+    std::vector<int32_t> order(N);
+    for (int i = 0; i < N; i++) order[i] = i;
+    auto key = [&](int32_t i) { const int x = i / (g.Y * g.Z), y = (i / g.Z) % g.Y; return (x + 4 * y) % g.X; };
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key(a) < key(b); });
+    greedy(order);
+    std::vector<int> best_col = col;
+    int best = (int)size.size();
+    uint32_t rng = 0x9E3779B9u;
+    for (int it = 0; it < 60; it++) {
+        // the classes of this pass, reordered: largest first, reversed, or shuffled
+        const int nc = (int)size.size();
+        std::vector<int> keys(nc);
+        for (int k = 0; k < nc; k++) keys[k] = k;
+        if (it % 3 == 0) std::stable_sort(keys.begin(), keys.end(), [&](int a, int b) { return size[a] > size[b]; });
+        else if (it % 3 == 1) std::reverse(keys.begin(), keys.end());
+        else
+            for (int k = nc - 1; k > 0; k--) {
+                rng = rng * 1664525u + 1013904223u;
+                std::swap(keys[k], keys[(int)((rng >> 8) % (uint32_t)(k + 1))]);
+            }
+        std::vector<std::vector<int32_t>> cls(nc);
+        for (int i = 0; i < N; i++) cls[col[i]].push_back(i);
+        order.clear();
+        for (int k : keys) order.insert(order.end(), cls[k].begin(), cls[k].end());
+        greedy(order);
+        if ((int)size.size() < best) { best = (int)size.size(); best_col = col; }
+    }
+    std::vector<std::vector<int32_t>> by(best);
+    for (int i = 0; i < N; i++) by[best_col[i]].push_back(i);
+    sites.clear();
+    start.assign(1, 0);
+    for (const auto &v : by) {
+        sites.insert(sites.end(), v.begin(), v.end());
+        start.push_back((int32_t)sites.size());
+    }
+    return best;
+}
+
+bool valid_torus_classes(const ReplicaGeom &g, int rc, const std::vector<int32_t> &sites,
+                         const std::vector<int32_t> &start)
+{
+    const int N = g.X * g.Y * g.Z;
+    if ((int)sites.size() != N || start.empty() || start.front() != 0 || start.back() != N) return false;
+    std::vector<int> col(N, -1);
+    for (size_t k = 0; k + 1 < start.size(); k++) {
+        if (start[k + 1] < start[k]) return false;
+        for (int32_t j = start[k]; j < start[k + 1]; j++) {
+            const int32_t s = sites[j];
+            if (s < 0 || s >= N || col[s] >= 0) return false;
+            col[s] = (int)k;
+        }
+    }
+    const std::vector<std::vector<int32_t>> nb = torus_neighbours(g, rc);
+    for (int s = 0; s < N; s++)
+        for (int32_t t : nb[s])
+            if (t != s && col[t] == col[s]) return false;
+    return true;
+}
+
 }  // namespace sn

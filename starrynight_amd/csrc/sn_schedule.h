@@ -100,4 +100,20 @@ bool periodic_sublattice(const std::vector<ReplicaGeom> &reps, int rc, int max_i
 void torus_classes(const Sublattice &lat, const ReplicaGeom &g, std::vector<int32_t> &sites,
                    std::vector<int32_t> &start);
 
+// Greedy colouring of one periodic lattice, for sizes whose sublattice
+// colourings need many colours (25 x 25 admits only index 25, 125 or 625):
+// sites within rc of each other through the periodic boundary get different
+// colours, and no colour holds more than max_class sites.  Iterated greedy
+// (J. Culberson, "Iterated greedy graph coloring", 1992): each pass colours
+// the sites class by class in a reordering of the previous pass's classes,
+// which never needs more colours.  sites / start as torus_classes; returns the
+// colour count.
+int greedy_torus_classes(const ReplicaGeom &g, int rc, int max_class, std::vector<int32_t> &sites,
+                         std::vector<int32_t> &start);
+
+// True if no two sites of one colour are within rc (periodic wrap) and every
+// site appears exactly once.
+bool valid_torus_classes(const ReplicaGeom &g, int rc, const std::vector<int32_t> &sites,
+                         const std::vector<int32_t> &start);
+
 }  // namespace sn

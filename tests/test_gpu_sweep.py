@@ -54,6 +54,11 @@ CASES = {
     "25x25x1_torus_lanes16": (dict(X=25, Y=25, Z=1, T=300, Efield=(0.0, 0.02, 0.0)), True,
                               dict(kernel=3, lanes_per_site=16)),
     "16cube_torus_lanes32": (dict(X=16, Y=16, Z=16, T=300), True, dict(kernel=3, lanes_per_site=32)),
+    # greedy periodic colourings (fewer colours than any sublattice of these sizes)
+    "30x26x1_torus_greedy_mixed_K": (dict(X=30, Y=26, Z=1, T=250, K=0.2, Efield=(0.02, 0.0, 0.0)), True,
+                                     dict(kernel=3)),
+    "11x13x1_torus_greedy": (dict(X=11, Y=13, Z=1, T=300), False, dict(kernel=3)),
+    "25x25x1_torus_lanes8": (dict(X=25, Y=25, Z=1, T=300), True, dict(kernel=3, lanes_per_site=8)),
 }
 
 
@@ -261,3 +266,26 @@ def test_torus_batch_and_limits(sn, oracle):
         with pytest.raises(sn.SnError) as e:
             sn.StarryNight(sn.ReplicaParams(T=300, **bad), sn.TuningParams(kernel=3))
         assert e.value.status == 5
+
+
+def test_torus_greedy_colouring(sn):
+    """25 x 25 at DipoleCutOff 3: every sublattice colouring holding the box has
+    25 colours (its index divides 625); the greedy colouring needs at most 16,
+    and no two sites of a colour are within the cutoff through the periodic
+    boundary."""
+    rp = sn.ReplicaParams(25, 25, 1, T=300)
+    print_case("torus_greedy_colouring", X=25, Y=25, Z=1)
+    with sn.StarryNight(rp, sn.TuningParams(kernel=3)) as ctx:
+        order, starts = ctx.schedule(0)
+        cfg = ctx.sweep_config()
+    order, starts = np.asarray(order), np.asarray(starts)
+    assert cfg["fine_colours"] <= 16 and len(starts) - 1 == cfg["fine_colours"]
+    assert np.array_equal(np.sort(order), np.arange(625))
+    x, y = order // 25, order % 25
+    for k in range(len(starts) - 1):
+        cx, cy = x[starts[k]:starts[k + 1]], y[starts[k]:starts[k + 1]]
+        dx = np.abs(cx[:, None] - cx[None, :]); dx = np.minimum(dx, 25 - dx)
+        dy = np.abs(cy[:, None] - cy[None, :]); dy = np.minimum(dy, 25 - dy)
+        d2 = dx * dx + dy * dy
+        np.fill_diagonal(d2, 100)
+        assert d2.min() > 9, f"colour {k}: sites within the cutoff"
