@@ -33,13 +33,18 @@ struct TorusArgs {
                                // runs of at most this many sites
 };
 
+// (c + d) mod L for 0 <= c < L and |d| < L, without branches: of c + d and
+// c + d + s (s = -L for d >= 0, +L for d < 0) one lies in [0, L) and the other
+// is negative or >= L, so as unsigned values the smaller one is the answer.
+__device__ __forceinline__ int wrap_pm(int t, int s) { return (int)min((unsigned)t, (unsigned)(t + s)); }
+
 template <int G, int NPL, bool MIXED, int BLK>
 __global__ __launch_bounds__(BLK) void torus_sweep_kernel(TorusArgs t, int nsweeps)
 {
     extern __shared__ double smem[];
     const int tid = threadIdx.x, r = blockIdx.x;
     const RepDev rep = t.reps[r];
-    const int X = rep.X, Y = rep.Y, Z = rep.Z, N = X * Y * Z;
+    const int X = rep.X, Y = rep.Y, Z = rep.Z, N = X * Y * Z, YZ = Y * Z;   // N < 2^24 (LDS)
     const int MS = t.max_sites, PS = t.prop_cap;
     double *sx = smem, *sy = sx + MS, *sz = sy + MS, *sl = sz + MS;
     double *prop = MIXED ? sl + MS : sl;    // 4 x PS: px, py, pz, threshold of the colour's sites
@@ -59,7 +64,7 @@ __global__ __launch_bounds__(BLK) void torus_sweep_kernel(TorusArgs t, int nswee
     // ---- this lane's share of the +-r pairs ----
     const int part = tid % G, group = tid / G;
     constexpr int NG = BLK / G;
-    int pdx[NPL], pdy[NPL], pdz[NPL];
+    int pdx[NPL], pdy[NPL], pdz[NPL], swx[NPL], swy[NPL], swz[NPL];   // offsets, wrap steps of +d
     double nx[NPL], ny[NPL], nz[NPL], w[NPL], w3[NPL];
 #pragma unroll
     for (int m = 0; m < NPL; m++) {
@@ -67,9 +72,10 @@ __global__ __launch_bounds__(BLK) void torus_sweep_kernel(TorusArgs t, int nswee
         if (j < t.npairs) {
             const PairDev p = t.pairs[j];
             pdx[m] = p.dx; pdy[m] = p.dy; pdz[m] = p.dz;
+            swx[m] = p.dx >= 0 ? -X : X; swy[m] = p.dy >= 0 ? -Y : Y; swz[m] = p.dz >= 0 ? -Z : Z;
             nx[m] = p.nx; ny[m] = p.ny; nz[m] = p.nz; w[m] = p.w; w3[m] = -3.0 * p.w;
         } else {
-            pdx[m] = 0; pdy[m] = 0; pdz[m] = 0;
+            pdx[m] = 0; pdy[m] = 0; pdz[m] = 0; swx[m] = -X; swy[m] = -Y; swz[m] = -Z;
             nx[m] = 0.0; ny[m] = 0.0; nz[m] = 0.0; w[m] = 0.0; w3[m] = 0.0;
         }
     }
@@ -91,7 +97,7 @@ __global__ __launch_bounds__(BLK) void torus_sweep_kernel(TorusArgs t, int nswee
             for (int i = tid; i < cs[k1] - p0; i += BLK) {
                 double px, py, pz, u;
                 const int pk = sites[p0 + i];
-                const int si = ((pk & 1023) * Y + ((pk >> 10) & 1023)) * Z + (pk >> 20);
+                const int si = __umul24(pk & 1023, YZ) + __umul24((pk >> 10) & 1023, Z) + (pk >> 20);
                 counter_draw(rep.seed, (uint32_t)r, sweep, (unsigned)si, rep.constrain_to_x, px, py, pz, u);
                 prop[i] = px;
                 prop[PS + i] = py;
@@ -107,21 +113,18 @@ __global__ __launch_bounds__(BLK) void torus_sweep_kernel(TorusArgs t, int nswee
                 if (q < c1) {   // whole groups (DPP rows) are active or not
                     const int pk = sites[q];
                     const int x = pk & 1023, y = (pk >> 10) & 1023, z = pk >> 20;
-                    const int si = (x * Y + y) * Z + z;
+                    const int si = __umul24(x, YZ) + __umul24(y, Z) + z;
                     double hx_, hy_, hz_, gx_ = 0.0, gy_ = 0.0, gz_ = 0.0;
                     double ax[NPL], ay[NPL], az[NPL], bx[NPL], by[NPL], bz[NPL], la[NPL], lb[NPL];
 #pragma unroll
                     for (int m = 0; m < NPL; m++) {
-                        // periodic wrap of x +- dx etc. (|d| <= rc < length)
-                        int xa = x + pdx[m], ya = y + pdy[m], za = z + pdz[m];
-                        int xb = x - pdx[m], yb = y - pdy[m], zb = z - pdz[m];
-                        xa -= xa >= X ? X : 0; xa += xa < 0 ? X : 0;
-                        ya -= ya >= Y ? Y : 0; ya += ya < 0 ? Y : 0;
-                        za -= za >= Z ? Z : 0; za += za < 0 ? Z : 0;
-                        xb -= xb >= X ? X : 0; xb += xb < 0 ? X : 0;
-                        yb -= yb >= Y ? Y : 0; yb += yb < 0 ? Y : 0;
-                        zb -= zb >= Z ? Z : 0; zb += zb < 0 ? Z : 0;
-                        const int ia = (xa * Y + ya) * Z + za, ib = (xb * Y + yb) * Z + zb;
+                        // periodic wrap of x +- dx etc. (|d| <= rc < length; -d wraps
+                        // the other way, so its step is -s)
+                        const int xa = wrap_pm(x + pdx[m], swx[m]), xb = wrap_pm(x - pdx[m], -swx[m]);
+                        const int ya = wrap_pm(y + pdy[m], swy[m]), yb = wrap_pm(y - pdy[m], -swy[m]);
+                        const int za = wrap_pm(z + pdz[m], swz[m]), zb = wrap_pm(z - pdz[m], -swz[m]);
+                        const int ia = __umul24(xa, YZ) + __umul24(ya, Z) + za;
+                        const int ib = __umul24(xb, YZ) + __umul24(yb, Z) + zb;
                         ax[m] = sx[ia]; ay[m] = sy[ia]; az[m] = sz[ia];
                         bx[m] = sx[ib]; by[m] = sy[ib]; bz[m] = sz[ib];
                         if (MIXED) { la[m] = sl[ia]; lb[m] = sl[ib]; }
