@@ -282,13 +282,14 @@ static size_t torus_lds(const sn_ctx *c, bool mixed)
 
 // lanes per site, pairs per lane and threads for npairs pairs and colours of
 // at most mc sites: 1024 threads when a colour has enough sites to keep them
-// busy (16 lanes x up to 64 sites for the greedy colourings, wide16)
-static bool torus_shape(int np, int mc, bool wide16, int &G, int &NPL, int &blk)
+// busy.  The greedy colourings' colours of up to 64 sites (wide) take 8 lanes
+// x 2 pairs in 512 threads (measured at 25 x 25: 4.3e7 site-updates/s against
+// 4.1e7 with 16 lanes in 1024 threads).
+static bool torus_shape(int np, int mc, bool wide, int &G, int &NPL, int &blk)
 {
     blk = 256;
-    if (np <= 16 && mc * 16 <= (wide16 ? 1024 : 512)) {
-        G = 16; NPL = 1; blk = mc * 16 > 512 ? 1024 : mc * 16 > 256 ? 512 : 256;
-    }
+    if (np <= 16 && mc * 16 <= 512) { G = 16; NPL = 1; blk = mc * 16 > 256 ? 512 : 256; }
+    else if (np <= 16 && wide && mc * 8 <= 512) { G = 8; NPL = 2; blk = 512; }
     else if (np <= 16) { G = 4; NPL = 4; }
     else if (np <= 32) { G = 8; NPL = 4; }
     else if (np <= 64) {
