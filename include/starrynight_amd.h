@@ -60,9 +60,11 @@ typedef struct sn_tuning {
     int32_t kernel;           /* 0 auto, 1 field-update tile kernel, 2 per-phase gather tile kernel
                                  (both visit sites in the same order with the same draws),
                                  3 whole-lattice sweeps of small replicas: one workgroup per replica
-                                 holds it in LDS, colours of a periodic sublattice, many sweeps per
-                                 launch (refused when a replica does not fit or no periodic
-                                 colouring exists; sn_schedule_order reports its order) */
+                                 holds it in LDS, colours of a periodic sublattice or (up to 16
+                                 neighbour pairs) a greedy periodic colouring, whichever needs
+                                 fewer steps; many sweeps per launch (refused when a replica does
+                                 not fit or no periodic colouring exists; sn_schedule_order
+                                 reports its order) */
     int32_t block_threads;    /* gather kernel's threads per tile: 0 or 256, or 512 */
 } sn_tuning;
 
