@@ -344,6 +344,11 @@ static int setup_torus(sn_ctx *c)
         go.resize(c->R); gs.resize(c->R);
         bool ok = true;
         for (int r = 0; r < c->R && ok; r++) {
+            int same = -1;   // replicas of one box share its colouring
+            for (int q = 0; q < r && same < 0; q++)
+                if (c->geom[q].X == c->geom[r].X && c->geom[q].Y == c->geom[r].Y && c->geom[q].Z == c->geom[r].Z)
+                    same = q;
+            if (same >= 0) { go[r] = go[same]; gs[r] = gs[same]; continue; }
             greedy_torus_classes(c->geom[r], c->rc, 64, go[r], gs[r]);
             ok = valid_torus_classes(c->geom[r], c->rc, go[r], gs[r]);
         }

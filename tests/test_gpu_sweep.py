@@ -289,3 +289,19 @@ def test_torus_greedy_colouring(sn):
         d2 = dx * dx + dy * dy
         np.fill_diagonal(d2, 100)
         assert d2.min() > 9, f"colour {k}: sites within the cutoff"
+
+
+def test_torus_greedy_batch(sn, oracle):
+    """Whole-lattice sweeps of a batch of uneven 2D replicas with greedy
+    colourings (two replicas share a box and its colouring; shorter colourings
+    are padded with empty colours), against the oracle."""
+    sizes = ((25, 25), (16, 20), (25, 25), (9, 7))
+    reps = [sn.ReplicaParams(X, Y, 1, T=200 + 50 * i, seed=90 + i) for i, (X, Y) in enumerate(sizes)]
+    print_case("torus_greedy_batch", sizes=sizes)
+    rng = np.random.default_rng(13)
+    lats = [rand_lattice(rp.N, rng, mixed=True) for rp in reps]
+    run_pair(sn, oracle, reps, lats, dict(kernel=3), nsweeps=4)
+    with sn.StarryNight(reps, sn.TuningParams(kernel=3)) as ctx:
+        o0, s0 = ctx.schedule(0)
+        o2, s2 = ctx.schedule(2)
+        assert np.array_equal(o0, o2) and np.array_equal(s0, s2)
