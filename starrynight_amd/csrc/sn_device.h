@@ -34,13 +34,16 @@ struct PairDev {
     double nx, ny, nz, w;  // unit vector r/|r| and 1/|r|^3
 };
 
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Each round's two
+// 32 x 32 -> 64-bit products are formed as 64-bit multiplies (one
+// v_mad_u64_u32 each) rather than separate high and low multiplies.
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
         c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
         k0 += 0x9E3779B9u;
