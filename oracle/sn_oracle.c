@@ -355,11 +355,29 @@ int64_t or_mc_moves_f64(const or_model *m, const or_neighbour *nb, int nnb, doub
     return moves;
 }
 
+/* the disc point of candidate words (a0, a1), if it lies in the disc */
+static int disc_point(uint32_t a0, uint32_t a1, double p[3])
+{
+    int64_t a = 2 * (int64_t)(a0 >> 6) + 1 - ((int64_t)1 << 26);
+    int64_t b = 2 * (int64_t)(a1 >> 6) + 1 - ((int64_t)1 << 26);
+    int64_t s = a * a + b * b;
+    if (s >= ((int64_t)1 << 52)) return 0;
+    double x1 = (double)a * (1.0 / 67108864.0);
+    double x2 = (double)b * (1.0 / 67108864.0);
+    double S = (double)s * (1.0 / 4503599627370496.0);
+    double r = sqrt(1.0 - S);
+    p[0] = (2.0 * x1) * r;
+    p[1] = (2.0 * x2) * r;
+    p[2] = 1.0 - 2.0 * S;
+    return 1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Counter-RNG draw for one site visit (the GPU sweep's proposal and u).
  * Philox4x32-10 with key = seed, counter = (site, sweep, replica, k).
  * u = 53-bit uniform from words 2,3 of k = 0.  Marsaglia (1972) disc point
- * from words 0,1 of k = 0, 1, 2, ... : x = a * 2^-26 with a odd in
+ * from words 0,1 of k = 0, then words 0,1 and 2,3 of k = 1, 2, ... (two
+ * candidates per retry call): x = a * 2^-26 with a odd in
  * (-2^26, 2^26), so the disc test a^2 + b^2 < 2^52 is exact in integers and
  * 1 - s, 1 - 2s are exact in double; the only rounded steps are one sqrt and
  * two multiplies, which makes the draw bit-reproducible on any IEEE machine. */
@@ -377,26 +395,14 @@ void or_counter_draw(uint64_t seed, uint32_t replica, uint32_t sweep, uint32_t s
         p[0] = ax[i][0]; p[1] = ax[i][1]; p[2] = ax[i][2];
         return;
     }
-    uint32_t a0 = w[0], a1 = w[1];
-    for (uint32_t k = 1; k <= 64; k++) {
-        int64_t a = 2 * (int64_t)(a0 >> 6) + 1 - ((int64_t)1 << 26);
-        int64_t b = 2 * (int64_t)(a1 >> 6) + 1 - ((int64_t)1 << 26);
-        int64_t s = a * a + b * b;
-        if (s < ((int64_t)1 << 52)) {
-            double x1 = (double)a * (1.0 / 67108864.0);
-            double x2 = (double)b * (1.0 / 67108864.0);
-            double S = (double)s * (1.0 / 4503599627370496.0);
-            double r = sqrt(1.0 - S);
-            p[0] = (2.0 * x1) * r;
-            p[1] = (2.0 * x2) * r;
-            p[2] = 1.0 - 2.0 * S;
-            return;
-        }
+    /* candidates: (w0, w1) of k = 0, then (w0, w1) and (w2, w3) of k = 1, 2, ... */
+    if (disc_point(w[0], w[1], p)) return;
+    for (uint32_t k = 1; k <= 32; k++) {
         ctr[3] = k;
         or_philox4x32_10(ctr, key, w);
-        a0 = w[0]; a1 = w[1];
+        if (disc_point(w[0], w[1], p) || disc_point(w[2], w[3], p)) return;
     }
-    p[0] = 0.0; p[1] = 0.0; p[2] = 1.0; /* 64 rejections in a row: p ~ 1e-43 */
+    p[0] = 0.0; p[1] = 0.0; p[2] = 1.0; /* 65 rejections in a row: p ~ 1e-43 */
 }
 
 void or_sweep_ordered_f64(const or_model *m, const or_neighbour *nb, int nnb, double *lat,

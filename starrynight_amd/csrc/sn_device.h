@@ -53,9 +53,11 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 
 // Proposal and acceptance uniform for one visit of `site` in `sweep`:
 // counter (site, sweep, replica, k), key = seed.  u = 53-bit uniform from words
-// 2, 3 of k = 0; Marsaglia (1972) disc point from words 0, 1 of k = 0, 1, ...
-// with x = a 2^-26, a odd: the disc test is exact in integers and 1 - s,
-// 1 - 2 s are exact, so only one sqrt and two products round.
+// 2, 3 of k = 0; Marsaglia (1972) disc point from words 0, 1 of k = 0, then
+// words 0, 1 and 2, 3 of k = 1, 2, ... (two candidates per retry call: a
+// wave waits for its slowest lane, so fewer calls per rejection shorten every
+// draw), with x = a 2^-26, a odd: the disc test is exact in integers and
+// 1 - s, 1 - 2 s are exact, so only one sqrt and two products round.
 __device__ __forceinline__ void counter_draw(unsigned long long seed, uint32_t replica,
                                              uint32_t sweep, uint32_t site, int constrain_to_x,
                                              double &px, double &py, double &pz, double &u)
@@ -71,26 +73,28 @@ __device__ __forceinline__ void counter_draw(unsigned long long seed, uint32_t r
         pz = (i == 4) ? 1.0 : (i == 5) ? -1.0 : 0.0;
         return;
     }
-    uint32_t a0 = w[0], a1 = w[1];
-    for (uint32_t k = 1; k <= 64; k++) {
-        long long a = 2 * (long long)(a0 >> 6) + 1 - (1ll << 26);
-        long long b = 2 * (long long)(a1 >> 6) + 1 - (1ll << 26);
-        long long s = a * a + b * b;
-        if (s < (1ll << 52)) {
-            double x1 = (double)a * (1.0 / 67108864.0);
-            double x2 = (double)b * (1.0 / 67108864.0);
-            double S = (double)s * (1.0 / 4503599627370496.0);
-            double r = __builtin_sqrt(1.0 - S);
-            px = (2.0 * x1) * r;
-            py = (2.0 * x2) * r;
-            pz = 1.0 - 2.0 * S;
-            return;
-        }
+    // the point of candidate words (a0, a1) if it lies in the disc
+    auto disc = [&](uint32_t a0, uint32_t a1) -> bool {
+        const long long a = 2 * (long long)(a0 >> 6) + 1 - (1ll << 26);
+        const long long b = 2 * (long long)(a1 >> 6) + 1 - (1ll << 26);
+        const long long s = a * a + b * b;
+        if (s >= (1ll << 52)) return false;
+        const double x1 = (double)a * (1.0 / 67108864.0);
+        const double x2 = (double)b * (1.0 / 67108864.0);
+        const double S = (double)s * (1.0 / 4503599627370496.0);
+        const double r = __builtin_sqrt(1.0 - S);
+        px = (2.0 * x1) * r;
+        py = (2.0 * x2) * r;
+        pz = 1.0 - 2.0 * S;
+        return true;
+    };
+    if (disc(w[0], w[1])) return;
+    for (uint32_t k = 1; k <= 32; k++) {
         uint32_t c[4] = {site, sweep, replica, k};
         philox4x32_10(c, k0, k1);
-        a0 = c[0]; a1 = c[1];
+        if (disc(c[0], c[1]) || disc(c[2], c[3])) return;
     }
-    px = 0.0; py = 0.0; pz = 1.0;
+    px = 0.0; py = 0.0; pz = 1.0;   // 65 rejections in a row: p ~ 1e-43
 }
 
 // Metropolis threshold: exp(-beta dE) > u  <=>  dE < -ln(u)/beta, and the
