@@ -53,53 +53,61 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        P = C.POINTER
-        dp, fp, ip, u64p = P(C.c_double), P(C.c_float), P(C.c_int32), P(C.c_uint64)
-        L.or_mt_seed.argtypes = [P(MT), C.c_uint32]
-        L.or_mt_u32.argtypes = [P(MT)]; L.or_mt_u32.restype = C.c_uint32
-        L.or_mt_real1.argtypes = [P(MT)]; L.or_mt_real1.restype = C.c_double
-        L.or_mt_real2.argtypes = [P(MT)]; L.or_mt_real2.restype = C.c_double
-        L.or_mt_fill_u32.argtypes = [P(MT), C.c_int64, P(C.c_uint32)]
-        L.or_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
-        L.or_gen_neighbour.argtypes = [C.c_int, C.c_int, P(Neighbour), C.c_int]
-        L.or_gen_neighbour.restype = C.c_int
-        L.or_site_energy_f32.argtypes = [P(Model), P(Neighbour), C.c_int, fp, C.c_int, C.c_int,
-                                         C.c_int, fp, fp]
-        L.or_site_energy_f32.restype = C.c_double
-        L.or_random_sphere_point_f32.argtypes = [P(MT), fp]
-        L.or_random_X_point_f32.argtypes = [P(MT), fp]
-        L.or_init_lattice_random_f32.argtypes = [P(MT), C.c_int, C.c_int, C.c_int, fp]
-        L.or_solid_solution_f32.argtypes = [P(MT), C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp]
-        L.or_solid_solution_f32.restype = C.c_int
-        L.or_mc_moves_f32.argtypes = [P(Model), P(Neighbour), C.c_int, fp, P(MT), C.c_int64,
-                                      u64p, u64p, P(MoveRec)]
-        L.or_mc_moves_f32.restype = C.c_int64
-        L.or_site_energy_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, C.c_int, C.c_int,
-                                         C.c_int, dp, dp]
-        L.or_site_energy_f64.restype = C.c_double
-        L.or_site_energy_batch_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, C.c_int64, ip,
-                                               dp, dp, dp]
-        L.or_mc_moves_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, P(MT), C.c_int64,
-                                      u64p, u64p]
-        L.or_mc_moves_f64.restype = C.c_int64
-        L.or_counter_draw.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
-                                      dp, dp]
-        L.or_sweep_ordered_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, ip, C.c_int64,
-                                           C.c_uint64, C.c_uint32, C.c_uint32, u64p, u64p]
-        L.or_energy_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, dp]
-        L.or_energy_f64.restype = C.c_double
-        L.or_gradient_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, dp, dp]
-        L.or_local_energy_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, C.c_int, C.c_int,
-                                          C.c_int, dp]
-        L.or_local_energy_f64.restype = C.c_double
-        L.or_moment_sum.argtypes = [dp, C.c_int64, dp]
-        L.or_radial_order_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, P(C.c_int64)]
-        L.or_dipole_potential_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
-        L.or_dipole_efield_f64.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
-        L.or_recombination_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def load(path: str):
+    """A liboracle.so built from sn_oracle.c (the default one, or a test's
+    variant with a planted mistake), with its argument types declared."""
+    L = C.CDLL(path)
+    P = C.POINTER
+    dp, fp, ip, u64p = P(C.c_double), P(C.c_float), P(C.c_int32), P(C.c_uint64)
+    L.or_mt_seed.argtypes = [P(MT), C.c_uint32]
+    L.or_mt_u32.argtypes = [P(MT)]; L.or_mt_u32.restype = C.c_uint32
+    L.or_mt_real1.argtypes = [P(MT)]; L.or_mt_real1.restype = C.c_double
+    L.or_mt_real2.argtypes = [P(MT)]; L.or_mt_real2.restype = C.c_double
+    L.or_mt_fill_u32.argtypes = [P(MT), C.c_int64, P(C.c_uint32)]
+    L.or_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    L.or_gen_neighbour.argtypes = [C.c_int, C.c_int, P(Neighbour), C.c_int]
+    L.or_gen_neighbour.restype = C.c_int
+    L.or_site_energy_f32.argtypes = [P(Model), P(Neighbour), C.c_int, fp, C.c_int, C.c_int,
+                                     C.c_int, fp, fp]
+    L.or_site_energy_f32.restype = C.c_double
+    L.or_random_sphere_point_f32.argtypes = [P(MT), fp]
+    L.or_random_X_point_f32.argtypes = [P(MT), fp]
+    L.or_init_lattice_random_f32.argtypes = [P(MT), C.c_int, C.c_int, C.c_int, fp]
+    L.or_solid_solution_f32.argtypes = [P(MT), C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp]
+    L.or_solid_solution_f32.restype = C.c_int
+    L.or_species_walk_f32.argtypes = [C.c_float, C.c_int, fp]
+    L.or_species_walk_f32.restype = C.c_int
+    L.or_mc_moves_f32.argtypes = [P(Model), P(Neighbour), C.c_int, fp, P(MT), C.c_int64,
+                                  u64p, u64p, P(MoveRec)]
+    L.or_mc_moves_f32.restype = C.c_int64
+    L.or_site_energy_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, C.c_int, C.c_int,
+                                     C.c_int, dp, dp]
+    L.or_site_energy_f64.restype = C.c_double
+    L.or_site_energy_batch_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, C.c_int64, ip,
+                                           dp, dp, dp]
+    L.or_mc_moves_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, P(MT), C.c_int64,
+                                  u64p, u64p]
+    L.or_mc_moves_f64.restype = C.c_int64
+    L.or_counter_draw.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                                  dp, dp]
+    L.or_sweep_ordered_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, ip, C.c_int64,
+                                       C.c_uint64, C.c_uint32, C.c_uint32, u64p, u64p]
+    L.or_energy_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, dp]
+    L.or_energy_f64.restype = C.c_double
+    L.or_gradient_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, dp, dp]
+    L.or_local_energy_f64.argtypes = [P(Model), P(Neighbour), C.c_int, dp, C.c_int, C.c_int,
+                                      C.c_int, dp]
+    L.or_local_energy_f64.restype = C.c_double
+    L.or_moment_sum.argtypes = [dp, C.c_int64, dp]
+    L.or_radial_order_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, P(C.c_int64)]
+    L.or_dipole_potential_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+    L.or_dipole_efield_f64.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.or_recombination_f64.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+    return L
 
 
 def _p(a, ct):
@@ -266,6 +274,29 @@ def moment_sum(lat64) -> np.ndarray:
     M = np.empty(3)
     lib().or_moment_sum(_p(lat64, C.c_double), lat64.shape[0], _p(M, C.c_double))
     return M
+
+
+def sphere_points_f32(mt: MT19937, n: int) -> np.ndarray:
+    """n successive random_sphere_point calls (float32 [n, 3]) on `mt`."""
+    out = np.zeros((n, 4), np.float32)
+    for i in range(n):
+        lib().or_random_sphere_point_f32(C.byref(mt.s), _p(out[i], C.c_float))
+    return out[:, :3].copy()
+
+
+def x_points_f32(mt: MT19937, n: int) -> np.ndarray:
+    """n successive random_X_point calls (float32 [n, 3]) on `mt`."""
+    out = np.zeros((n, 4), np.float32)
+    for i in range(n):
+        lib().or_random_X_point_f32(C.byref(mt.s), _p(out[i], C.c_float))
+    return out[:, :3].copy()
+
+
+def species_walk_f32(sample: float, prevalence) -> int:
+    """solid_solution's walk of one float sample down the prevalence list:
+    the species index, or -1 past the end of the table."""
+    pv = np.ascontiguousarray(prevalence, np.float32)
+    return lib().or_species_walk_f32(C.c_float(sample), len(pv), _p(pv, C.c_float))
 
 
 def random_lattice_f32(X, Y, Z, seed, lengths=(1.0,), prevalence=(1.0,)):

@@ -205,7 +205,20 @@ void or_init_lattice_random_f32(or_mt19937 *g, int X, int Y, int Z, float *lat)
     for (int64_t i = 0; i < N; i++) or_random_sphere_point_f32(g, lat + 4 * i);
 }
 
-/* solid_solution (src/starrynight-lattice.c:139-171).  Returns -1 where the
+/* The species walk of solid_solution (src/starrynight-lattice.c:154-159):
+ * the float sample steps down the list while it exceeds the entry's
+ * prevalence, subtracting it in float.  Returns the species index, or -1
+ * where the reference would read past its n-entry table. */
+int or_species_walk_f32(float sample, int n, const float *prev)
+{
+    int i;
+    for (i = 0; sample > prev[i]; sample -= prev[i], i++)
+        if (i + 1 >= n) return -1;
+    return i;
+}
+
+/* solid_solution (src/starrynight-lattice.c:139-171): one genrand_real1 per
+ * site in x, y, z order, narrowed to float (:143, :152).  Returns -1 where the
  * reference would walk past the species list (prevalence summing below 1). */
 int or_solid_solution_f32(or_mt19937 *g, int X, int Y, int Z, int n, const float *len,
                           const float *prev, float *lat)
@@ -213,9 +226,8 @@ int or_solid_solution_f32(or_mt19937 *g, int X, int Y, int Z, int n, const float
     int64_t N = (int64_t)X * Y * Z;
     for (int64_t s = 0; s < N; s++) {
         float sample = or_mt_real1(g);
-        int i;
-        for (i = 0; sample > prev[i]; sample -= prev[i], i++)
-            if (i + 1 >= n) return -1;
+        int i = or_species_walk_f32(sample, n, prev);
+        if (i < 0) return -1;
         lat[4 * s + 3] = len[i];
     }
     return 0;

@@ -71,6 +71,7 @@ void or_random_X_point_f32(or_mt19937 *g, float p[3]);
 void or_init_lattice_random_f32(or_mt19937 *g, int X, int Y, int Z, float *lat);
 int  or_solid_solution_f32(or_mt19937 *g, int X, int Y, int Z, int n, const float *len,
                            const float *prev, float *lat);
+int  or_species_walk_f32(float sample, int n, const float *prev);
 
 typedef struct {
     int32_t site;      /* (x*Y + y)*Z + z */

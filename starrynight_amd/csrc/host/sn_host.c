@@ -24,7 +24,12 @@
  * src/starrynight-analysis.c:582-594; with several replicas the file names
  * carry "_r<replica>" before the extension.
  *
- * usage: sn_host [-c config] [T [CageStrain]]
+ * usage: sn_host [-c config] [--dry-run lattice.bin] [T [CageStrain]]
+ *
+ * --dry-run reads the config and the argv overrides, prints the resolved
+ * parameters of every replica (key=value lines on stdout), writes replica 0's
+ * reference initial lattice (double[N][4]) to the named file and exits before
+ * any device call.
  */
 #include <math.h>
 #include <stdint.h>
@@ -212,9 +217,13 @@ static int load(const char *path, snc_config *cf, run_cfg *r)
 
 int main(int argc, char **argv)
 {
-    const char *path = "starrynight.cfg";
+    const char *path = "starrynight.cfg", *dry_run = NULL;
     int ai = 1;
-    if (argc > 2 && strcmp(argv[1], "-c") == 0) { path = argv[2]; ai = 3; }
+    while (argc > ai + 1 && (strcmp(argv[ai], "-c") == 0 || strcmp(argv[ai], "--dry-run") == 0)) {
+        if (strcmp(argv[ai], "-c") == 0) path = argv[ai + 1];
+        else dry_run = argv[ai + 1];
+        ai += 2;
+    }
     snc_config cf;
     run_cfg r;
     fprintf(stderr, "Starry Night (MI355X) - Monte Carlo brushstrokes.\n");
@@ -253,6 +262,28 @@ int main(int argc, char **argv)
         ps[i].K = r.K; ps[i].constrain_to_x = r.constrain_x; ps[i].temperature = Ti;
         ps[i].seed = have_seed ? (uint64_t)seed_base + (uint64_t)i
                                : (uint64_t)(uint32_t)(0xDEADBEEFu + (uint32_t)(int)Ti) + ((uint64_t)(i / nT) << 32);
+    }
+    if (dry_run) {
+        /* resolved parameters, then replica 0's initial lattice; no device call */
+        printf("T=%d\nCageStrain=%.17g\nX=%d\nY=%d\nZ=%d\nDipoleCutOff=%d\nInitialLattice=%s\nreplicas=%d\n", r.T,
+               r.J, r.X, r.Y, r.Z, r.rc, r.initial, R);
+        for (int i = 0; i < R; i++)
+            printf("replica=%d temperature=%.17g seed=%llu mt_seed=%u efield=%.9g,%.9g,%.9g\n", i, ps[i].temperature,
+                   (unsigned long long)ps[i].seed, (unsigned)(0xDEADBEEFu + (uint32_t)(int)Ts[i % nT]),
+                   ps[i].efield[0], ps[i].efield[1], ps[i].efield[2]);
+        long long Nd = (long long)r.X * r.Y * r.Z;
+        double *l0 = (double *)malloc(sizeof(double) * 4 * Nd);
+        if (!l0) return 1;
+        int st0 = sn_reference_initial_lattice(r.X, r.Y, r.Z, r.initial, (uint32_t)(0xDEADBEEFu + (uint32_t)(int)Ts[0]),
+                                               r.nspec, r.len, r.prev, l0);
+        if (st0 != SN_OK) { fprintf(stderr, "initial lattice: %s\n", sn_status_string(st0)); return 1; }
+        FILE *f = fopen(dry_run, "wb");
+        if (!f || fwrite(l0, sizeof(double) * 4, Nd, f) != (size_t)Nd) { fprintf(stderr, "cannot write %s\n", dry_run); return 1; }
+        fclose(f);
+        free(l0);
+        free(ps);
+        snc_destroy(&cf);
+        return 0;
     }
     sn_tuning tun;
     memset(&tun, 0, sizeof(tun));

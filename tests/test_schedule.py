@@ -23,6 +23,14 @@ def offsets(rc, two_d):
 
 def check_schedule(sn, rp, tuning=None, coarse=None):
     order, starts = sn.plan_schedule(rp, 0, tuning)
+    return check_order(rp, order, starts, coarse)
+
+
+def check_order(rp, order, starts, coarse=None):
+    """A visit order (a permutation grouped into phases) is conflict-free:
+    no two sites of one phase within DipoleCutOff through the periodic
+    boundary.  Returns the number of phases."""
+    order, starts = np.asarray(order), np.asarray(starts)
     N = rp.N
     assert len(order) == N and np.array_equal(np.sort(order), np.arange(N))
     assert starts[0] == 0 and starts[-1] == N and np.all(np.diff(starts) >= 0)
@@ -101,3 +109,16 @@ def test_plan_schedule_rejects_bad_parameters(sn):
         sn.plan_schedule(sn.ReplicaParams(16, 16, 16, DipoleCutOff=0))
     with pytest.raises(sn.SnError):
         sn.plan_schedule(sn.ReplicaParams(9, 9, 9, DipoleCutOff=3))   # untileable at rc = 3
+
+
+@pytest.mark.parametrize("dims,rc,tun", [
+    ((128, 128, 128), 3, dict(tile=(8, 8, 8), fine_colouring=3, kernel=2)),   # the headline sweep
+    ((64, 64, 64), 3, dict(tile=(8, 8, 8), fine_colouring=3, kernel=2)),
+    ((128, 128, 128), 4, dict(tile=(8, 8, 10), fine_colouring=2, kernel=2)),  # solid-solution ramp
+], ids=["128cube_rc3_bcc_8", "64cube_rc3_bcc_8", "128cube_rc4_minidx_8x8x10"])
+def test_benchmark_schedules_are_conflict_free(sn, dims, rc, tun):
+    """The schedules the benchmark times at full size, checked site by site
+    through the periodic boundary."""
+    rp = sn.ReplicaParams(*dims, DipoleCutOff=rc)
+    nph = check_schedule(sn, rp, sn.TuningParams(**tun), coarse=8)
+    assert nph == 8 * (32 if rc == 3 else 54)
