@@ -88,37 +88,87 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu_info():
+    """The host CPU as lscpu reports it, and how many CPUs this process may
+    use (scheduler affinity, cgroup CPU quota)."""
+    info = {"model": None, "sockets": None, "cores_per_socket": None, "threads_per_core": None}
+    try:
+        for line in subprocess.check_output(["lscpu"], text=True, env={**os.environ, "LC_ALL": "C"}).splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        pass
+    if info["sockets"] and info["cores_per_socket"]:
+        info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity_cpus"] = os.cpu_count() or 1
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    return info
+
+
+# CPUs a benchmark may keep busy beside one GPU on the shared GPU hosts
+CPU_SHARE_PER_GPU = 16
+
+
 def cpu_baseline(seconds_target: float = 10.0, procs: int | None = None):
-    """Time oracle/sn_cpu_bench (float-faithful port of MC_moves, gcc -O3,
-    one thread per process) on the 128^3 rc=3 workload: one process alone,
-    then `procs` processes at once (distinct T, as the reference's GNU-parallel
-    runs).  Sample: a bounded number of moves per process."""
+    """Time oracle/sn_cpu_bench on the 128^3 rc=3 workload: the float-faithful
+    port of MC_moves (gcc -O3, one thread per process) on one core, then one
+    process per usable physical core at once (distinct T, as the reference's
+    GNU-parallel runs), and the FP64 restatement on one core.  Sample: a
+    bounded number of moves per process (about `seconds_target` s each)."""
     exe = os.path.join(ROOT, "oracle", "sn_cpu_bench")
     if not os.path.exists(exe):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"])
+    host = host_cpu_info()
+
+    def run(*args):
+        return json.loads(subprocess.check_output([exe, *map(str, args)]))
+
     # calibrate: 0.5 M moves
-    out = json.loads(subprocess.check_output([exe, "128", "128", "128", "3", "300", "500000"]))
-    rate1 = out["rate"]
+    rate1 = run(128, 128, 128, 3, 300, 500000)["rate"]
     moves = int(max(1e6, rate1 * seconds_target))
-    single = json.loads(subprocess.check_output([exe, "128", "128", "128", "3", "300", str(moves)]))
+    single = run(128, 128, 128, 3, 300, moves)
     if procs is None:
-        try:
-            procs = len(os.sched_getaffinity(0))
-        except AttributeError:
-            procs = os.cpu_count() or 1
-        procs = max(1, min(procs, 16))
+        usable = [host.get("physical_cores") or host["affinity_cpus"], host["affinity_cpus"], CPU_SHARE_PER_GPU]
+        if host["cgroup_cpu_quota"]:
+            usable.append(int(host["cgroup_cpu_quota"]))
+        procs = max(1, min(usable))
     ps = [subprocess.Popen([exe, "128", "128", "128", "3", str(300 + 10 * i), str(moves)],
                            stdout=subprocess.PIPE) for i in range(procs)]
     rates = [json.loads(p.communicate()[0])["rate"] for p in ps]
+    f64_moves = int(max(5e5, rate1 * seconds_target / 2))
+    f64 = run(128, 128, 128, 3, 300, f64_moves, "f64")
     # the 2D 25x25x1 rc=3 case (the reference's CPU-runnable configuration), one core, 3000 sweeps
-    sq = json.loads(subprocess.check_output([exe, "25", "25", "1", "3", "300", str(3000 * 625)]))
+    sq = run(25, 25, 1, 3, 300, 3000 * 625)
     return {
         "value": sum(rates), "unit": "site-updates/s", "cores": procs, "kind": "port",
         "single_core": single["rate"],
+        "single_core_fp64_oracle": f64["rate"],
         "square_25x25_rc3_single_core": sq["rate"],
+        "host": host,
         "sample": f"128^3 rc=3 T=300 random lattice, {moves} moves per process "
                   f"(~{moves / single['rate']:.0f} s), float-faithful restatement of "
-                  f"MC_moves/site_energy with MT19937, gcc -O3, {procs} processes",
+                  f"MC_moves/site_energy with MT19937, gcc -O3, {procs} single-threaded processes at once "
+                  f"(one per usable physical core: min of lscpu's physical cores, the affinity mask, the cgroup "
+                  f"quota and the {CPU_SHARE_PER_GPU} CPUs a GPU's job may use on this host); the FP64 "
+                  f"restatement (or_mc_moves_f64) on one core, {f64_moves} moves",
     }
 
 
@@ -210,14 +260,18 @@ def main():
     kname = "field_sweep_kernel" if field else "tile_sweep_kernel"
     traffic = measured_traffic(kname, L, rc)
     hbm_alg = updates_per_launch * BYTES_PER_UPDATE / (mean_ms * 1e-3) / 1e9
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    roof = {"bound": "fp64", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+            "traffic_source": (f"profiles/{kname}_{L}cube_rc{rc}_traffic.txt: rocprofv3 --pmc FETCH_SIZE and "
+                               "WRITE_SIZE of this kernel on this workload, scaled by a copy kernel's readings")
+            if traffic is not None else None,
             "kernel": kname, "mean_launch_ms": round(mean_ms, 5), "launches_timed": nl,
             "updates_per_launch": updates_per_launch, "flop_per_update": F,
             "algorithmic_bytes_per_update": BYTES_PER_UPDATE,
             "algorithmic_hbm_GBps": round(hbm_alg, 1), "hbm_peak_GBps": PEAK_HBM_GBS,
-            "bound_note": "compute-bound at the FP64 peak (78.6 TFLOP/s, the same for vector and "
-                          "matrix FP64 on MI355X); the kernel runs FP64 VALU, not MFMA (a stencil sum)"}
+            "bound_note": "compute-bound: FP64 vector ALU (VALU) peak, 78.6 TFLOP/s on MI355X; "
+                          "no MFMA (a stencil sum with a data-dependent update per phase, not a dense "
+                          "contraction); 42 flop per algorithmic byte, far right of the HBM ridge"}
     log(f"[bench] {kname} mean {mean_ms * 1e3:.1f} us/launch over {nl} launches -> "
         f"{achieved:.2f} TFLOP/s ({achieved / PEAK_FP64_TFLOPS:.1%} of FP64 peak), "
         f"{hbm_alg:.0f} GB/s algorithmic")

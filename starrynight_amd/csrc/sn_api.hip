@@ -66,6 +66,10 @@ struct sn_ctx {
     bool use_field = true;
     double *d_partials = nullptr, *d_result = nullptr;
     size_t partial_cap = 0;
+    // staging for the host-buffer entry points (site energies, replay, gradient
+    // copies, species tables): grows as needed, freed with the context
+    void *d_stage = nullptr;
+    size_t stage_cap = 0;
     int max_tile_sites = 0, max_cls = 0, max_chunks = 0;
     int field_region = 0;   // largest staged region of the field kernel (z rows padded to odd)
     int *d_tile_list = nullptr;          // tile indices grouped by replica (tiled energy kernel)
@@ -170,6 +174,47 @@ static int ensure_partials(sn_ctx *c, size_t n)
     c->partial_cap = n;
     return SN_OK;
 }
+
+// The staging buffer, at least `bytes` long.  Its contents do not survive a
+// call that grows it.
+static int ensure_stage(sn_ctx *c, size_t bytes)
+{
+    if (bytes <= c->stage_cap) return SN_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // no queued work may still read the old buffer
+    if (c->d_stage) (void)hipFree(c->d_stage);
+    c->d_stage = nullptr;
+    c->stage_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 1 << 16);
+    HIPCHK(c, hipMalloc(&c->d_stage, cap));
+    c->stage_cap = cap;
+    return SN_OK;
+}
+
+// Byte offsets of consecutive 256-byte-aligned slices of the staging buffer.
+struct StageLayout {
+    size_t end = 0;
+    size_t take(size_t bytes)
+    {
+        const size_t at = end;
+        end += (bytes + 255) & ~(size_t)255;
+        return at;
+    }
+};
+
+template <typename T>
+static T *stage_ptr(sn_ctx *c, size_t offset)
+{
+    return (T *)((char *)c->d_stage + offset);
+}
+
+// HIP events destroyed when the list goes out of scope (every return path).
+struct EventList {
+    std::vector<hipEvent_t> ev;
+    ~EventList()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
 
 // Column sums of the nblk x width partials table into d_result (at least 256
 // doubles), kReduceCols columns per workgroup.
@@ -670,7 +715,7 @@ int sn_destroy(sn_ctx *c)
                     c->d_offs, c->d_sweep,
                     c->d_counters, c->d_partials, c->d_result, c->d_radial, c->d_radial_bins,
                     c->d_stencil, c->d_obs, c->d_tile_list, c->d_tile_seg, c->d_torus_sites,
-                    c->d_torus_off, c->d_torus_cs, c->d_rep_counters};
+                    c->d_torus_off, c->d_torus_cs, c->d_rep_counters, c->d_stage};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_result) (void)hipHostFree(c->h_result);
@@ -790,18 +835,17 @@ int sn_init_lattice_random(sn_ctx *c, int32_t r, int32_t n_species, const double
         if (v != used[0]) h.mixed = 1;
     h.L0 = h.mixed ? 0.0 : used[0];
     if ((st = upload_rep(c, r))) return st;
-    double *dl = nullptr, *dc = nullptr;
-    HIPCHK(c, hipMalloc((void **)&dl, len.size() * sizeof(double)));
-    HIPCHK(c, hipMalloc((void **)&dc, cum.size() * sizeof(double)));
-    HIPCHK(c, hipMemcpy(dl, len.data(), len.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(dc, cum.data(), cum.size() * sizeof(double), hipMemcpyHostToDevice));
+    StageLayout lay;
+    const size_t o_len = lay.take(len.size() * sizeof(double)), o_cum = lay.take(cum.size() * sizeof(double));
+    if ((st = ensure_stage(c, lay.end))) return st;
+    double *dl = stage_ptr<double>(c, o_len), *dc = stage_ptr<double>(c, o_cum);
+    HIPCHK(c, hipMemcpyAsync(dl, len.data(), len.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dc, cum.data(), cum.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int nblk = (int)((h.N + 255) / 256);
     hipLaunchKernelGGL(init_random_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r,
                        (int)len.size(), dl, dc);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    (void)hipFree(dl);
-    (void)hipFree(dc);
     return SN_OK;
 }
 
@@ -842,20 +886,20 @@ int sn_site_energy_batch(sn_ctx *c, int32_t r, int64_t n, const int32_t *site, c
     if (!site || !prop || !dE) return fail(c, SN_ERR_INVALID, "null buffer");
     for (int64_t k = 0; k < n; k++)
         if (site[k] < 0 || site[k] >= c->reps[r].N) return fail(c, SN_ERR_RANGE, "site index out of range");
-    int32_t *ds = nullptr;
-    double *dp = nullptr, *de = nullptr;
-    HIPCHK(c, hipMalloc((void **)&ds, n * sizeof(int32_t)));
-    HIPCHK(c, hipMalloc((void **)&dp, n * 3 * sizeof(double)));
-    HIPCHK(c, hipMalloc((void **)&de, n * sizeof(double)));
-    HIPCHK(c, hipMemcpy(ds, site, n * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(dp, prop, n * 3 * sizeof(double), hipMemcpyHostToDevice));
-    st = sn_site_energy_batch_device(c, r, n, ds, dp, de);
-    if (st == SN_OK) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipMemcpy(dE, de, n * sizeof(double), hipMemcpyDeviceToHost));
-    }
-    (void)hipFree(ds); (void)hipFree(dp); (void)hipFree(de);
-    return st;
+    // inputs and outputs go through the context's staging buffer (no
+    // allocation once it has grown to the largest batch)
+    StageLayout lay;
+    const size_t o_p = lay.take(n * 3 * sizeof(double)), o_e = lay.take(n * sizeof(double)),
+                 o_s = lay.take(n * sizeof(int32_t));
+    if ((st = ensure_stage(c, lay.end))) return st;
+    double *dp = stage_ptr<double>(c, o_p), *de = stage_ptr<double>(c, o_e);
+    int32_t *ds = stage_ptr<int32_t>(c, o_s);
+    HIPCHK(c, hipMemcpyAsync(ds, site, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dp, prop, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((st = sn_site_energy_batch_device(c, r, n, ds, dp, de))) return st;
+    HIPCHK(c, hipMemcpyAsync(dE, de, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SN_OK;
 }
 
 static SweepArgs sweep_args(sn_ctx *c)
@@ -903,6 +947,17 @@ constexpr int kTorusSweeps = 256;
 
 // nsw whole-lattice sweeps of every replica in one launch, then the
 // sweep-counter bump; events (optional) bracket the sweep kernel.
+// Two new events, appended to *events as soon as each exists (so that the
+// owner of the list destroys them whatever fails later).
+static int event_pair(sn_ctx *c, std::vector<hipEvent_t> *events, hipEvent_t *e0, hipEvent_t *e1)
+{
+    HIPCHK(c, hipEventCreate(e0));
+    events->push_back(*e0);
+    HIPCHK(c, hipEventCreate(e1));
+    events->push_back(*e1);
+    return SN_OK;
+}
+
 static int launch_torus(sn_ctx *c, int nsw, std::vector<hipEvent_t> *events)
 {
     const bool mixed = any_mixed(c);
@@ -912,19 +967,12 @@ static int launch_torus(sn_ctx *c, int nsw, std::vector<hipEvent_t> *events)
     t.ncol = c->torus_ncol; t.sweep_ctr = c->d_sweep; t.counters = c->d_rep_counters;
     t.max_sites = c->torus_ms; t.prop_cap = c->torus_pc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (events) {
-        HIPCHK(c, hipEventCreate(&e0));
-        HIPCHK(c, hipEventCreate(&e1));
-        HIPCHK(c, hipEventRecord(e0, c->stream));
-    }
+    if (events && (event_pair(c, events, &e0, &e1) || hipEventRecord(e0, c->stream) != hipSuccess))
+        return fail(c, SN_ERR_HIP, "timing events");
     hipLaunchKernelGGL(pick_torus(c->torus_G, c->torus_NPL, mixed, c->torus_blk), dim3(c->R), dim3(c->torus_blk),
                        torus_lds(c, mixed), c->stream, t, nsw);
     HIPCHK(c, hipGetLastError());
-    if (events) {
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-        events->push_back(e0);
-        events->push_back(e1);
-    }
+    if (events) HIPCHK(c, hipEventRecord(e1, c->stream));
     hipLaunchKernelGGL(bump_sweep_kernel, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, c->d_sweep, c->R,
                        (unsigned)nsw);
     HIPCHK(c, hipGetLastError());
@@ -940,13 +988,10 @@ static int enqueue_sweep(sn_ctx *c, std::vector<hipEvent_t> *events)
         int st;
         if (events) {
             hipEvent_t e0, e1;
-            HIPCHK(c, hipEventCreate(&e0));
-            HIPCHK(c, hipEventCreate(&e1));
+            if ((st = event_pair(c, events, &e0, &e1))) return st;
             HIPCHK(c, hipEventRecord(e0, c->stream));
             if ((st = launch_coarse(c, mixed, first, n))) return st;
             HIPCHK(c, hipEventRecord(e1, c->stream));
-            events->push_back(e0);
-            events->push_back(e1);
         } else if ((st = launch_coarse(c, mixed, first, n))) return st;
     }
     hipLaunchKernelGGL(bump_sweep_kernel, dim3((c->R + 255) / 256), dim3(256), 0, c->stream, c->d_sweep, c->R, 1u);
@@ -1033,7 +1078,8 @@ int sn_time_sweep_kernel(sn_ctx *c, int32_t nsweeps, double *mean_ms, int32_t *l
 {
     if (!c || nsweeps < 1 || !mean_ms) return fail(c, SN_ERR_INVALID, "bad arguments");
     if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
-    std::vector<hipEvent_t> ev;
+    EventList evl;   // destroyed on every return path
+    std::vector<hipEvent_t> &ev = evl.ev;
     int st = SN_OK;
     for (int i = 0; i < nsweeps && st == SN_OK; i++) st = c->torus ? launch_torus(c, 1, &ev) : enqueue_sweep(c, &ev);
     if (st) return st;
@@ -1045,7 +1091,6 @@ int sn_time_sweep_kernel(sn_ctx *c, int32_t nsweeps, double *mean_ms, int32_t *l
         HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
         tot += ms;
     }
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     int nl = (int)(ev.size() / 2);
     *mean_ms = nl ? tot / nl : 0.0;
     if (launches) *launches = nl;
@@ -1064,37 +1109,32 @@ int sn_mc_replay(sn_ctx *c, int32_t r, int64_t n, const int32_t *site, const dou
         return fail(c, SN_ERR_INVALID, "null buffer");
     for (int64_t k = 0; k < n; k++)
         if (site[k] < 0 || site[k] >= c->reps[r].N) return fail(c, SN_ERR_RANGE, "site index out of range");
-    int32_t *ds = nullptr;
-    double *dp = nullptr, *du = nullptr;
-    uint8_t *dd = nullptr, *da = nullptr, *dout = nullptr;
-    unsigned long long *dstat = nullptr;
-    HIPCHK(c, hipMalloc((void **)&ds, n * sizeof(int32_t)));
-    HIPCHK(c, hipMalloc((void **)&dp, n * 3 * sizeof(double)));
-    HIPCHK(c, hipMalloc((void **)&du, n * sizeof(double)));
-    HIPCHK(c, hipMalloc((void **)&dd, n));
-    HIPCHK(c, hipMalloc((void **)&da, n));
-    HIPCHK(c, hipMalloc((void **)&dout, n));
-    HIPCHK(c, hipMalloc((void **)&dstat, 4 * sizeof(unsigned long long)));
-    HIPCHK(c, hipMemcpy(ds, site, n * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(dp, prop, n * 3 * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(du, u, n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(dd, u_drawn, n, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(da, ref_accept, n, hipMemcpyHostToDevice));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    StageLayout lay;
+    const size_t o_p = lay.take(n * 3 * sizeof(double)), o_u = lay.take(n * sizeof(double)),
+                 o_st = lay.take(4 * sizeof(unsigned long long)), o_s = lay.take(n * sizeof(int32_t)),
+                 o_d = lay.take(n), o_a = lay.take(n), o_o = lay.take(n);
+    if ((st = ensure_stage(c, lay.end))) return st;
+    double *dp = stage_ptr<double>(c, o_p), *du = stage_ptr<double>(c, o_u);
+    unsigned long long *dstat = stage_ptr<unsigned long long>(c, o_st);
+    int32_t *ds = stage_ptr<int32_t>(c, o_s);
+    uint8_t *dd = stage_ptr<uint8_t>(c, o_d), *da = stage_ptr<uint8_t>(c, o_a), *dout = stage_ptr<uint8_t>(c, o_o);
+    HIPCHK(c, hipMemcpyAsync(ds, site, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dp, prop, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(du, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dd, u_drawn, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(da, ref_accept, n, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(replay_kernel, dim3(1), dim3(256), 0, c->stream, c->d_lat, c->d_reps, r, c->d_offs,
                        c->noff, (long long)n, ds, dp, du, dd, da, dout, dstat);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     unsigned long long stats[4];
-    HIPCHK(c, hipMemcpy(decision_out, dout, n, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(stats, dstat, sizeof(stats), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(decision_out, dout, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(stats, dstat, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (mismatches) *mismatches = stats[0];
     if (near_ties) *near_ties = stats[1];
     // the replay's decisions count toward the replica's counters
     c->replay_counts[2 * r] += stats[2];
     c->replay_counts[2 * r + 1] += stats[3];
-    void *bufs[] = {ds, dp, du, dd, da, dout, dstat};
-    for (void *b : bufs) (void)hipFree(b);
     return SN_OK;
 }
 
@@ -1221,16 +1261,14 @@ int sn_gradient(sn_ctx *c, int32_t r, double *grad_p, double dparams[6], double 
     if (st) return st;
     const RepHost &h = c->reps[r];
     double *dg = nullptr;
-    if (grad_p) HIPCHK(c, hipMalloc((void **)&dg, h.N * 3 * sizeof(double)));
-    if ((st = run_energy(c, r, dg)) || (st = fetch_result(c, kNPart))) {
-        if (dg) (void)hipFree(dg);
-        return st;
-    }
     if (grad_p) {
-        const hipError_t e = hipMemcpy(grad_p, dg, h.N * 3 * sizeof(double), hipMemcpyDeviceToHost);
-        (void)hipFree(dg);
-        if (e != hipSuccess) return fail(c, SN_ERR_HIP, std::string("gradient copy: ") + hipGetErrorString(e));
+        if ((st = ensure_stage(c, (size_t)h.N * 3 * sizeof(double)))) return st;
+        dg = stage_ptr<double>(c, 0);
     }
+    if ((st = run_energy(c, r, dg))) return st;
+    if (grad_p)
+        HIPCHK(c, hipMemcpyAsync(grad_p, dg, h.N * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((st = fetch_result(c, kNPart))) return st;
     if (dparams) for (int i = 0; i < 6; i++) dparams[i] = c->h_result[4 + i];
     if (energy) *energy = energy_total(c->h_result);
     return SN_OK;

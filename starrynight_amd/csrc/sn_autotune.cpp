@@ -9,10 +9,11 @@
 // sn_create and are skipped.
 //
 // Cache: one line per configuration, "key<TAB>t0 t1 t2 lanes colouring graph
-// kernel block rate".  The key is the device architecture, DipoleCutOff and the
-// multiset of replica sizes (the lattice contents are not known at tuning
-// time, so mixed and uniform lengths share an entry).  Path: $SN_TUNE_CACHE,
-// else $HOME/.cache/starrynight_amd/tune.txt; no cache when neither is set.
+// kernel block rate".  The key is the device architecture, DipoleCutOff, whether
+// the species table gives mixed or uniform lengths (mixed lengths stage a fourth
+// LDS array, which changes the best tile), and the multiset of replica sizes.
+// Path: $SN_TUNE_CACHE, else $HOME/.cache/starrynight_amd/tune.txt; no cache
+// when neither is set.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -47,6 +47,11 @@ __global__ void site_energy_batch_kernel(const double4 *lat, const RepDev *reps,
     const RepDev rep = reps[replica];
     const double4 *L = lat + rep.base;
     int s = site[k];
+    // an index outside the replica reads nothing and yields NaN
+    if (s < 0 || (long long)s >= (long long)rep.X * rep.Y * rep.Z) {
+        dE[k] = __builtin_nan("");
+        return;
+    }
     int z = s % rep.Z, q = s / rep.Z, y = q % rep.Y, x = q / rep.Y;
     double4 o = L[s];
     FieldOut f;
