@@ -132,7 +132,9 @@ int sn_set_params(sn_ctx *ctx, int32_t replica, double temperature, const double
 int sn_site_energy_batch(sn_ctx *ctx, int32_t replica, int64_t n, const int32_t *site,
                          const double *prop, double *dE);
 /* Same with device pointers (site: int32[n], prop: double[n][3], dE: double[n]),
- * enqueued on the context stream without synchronising. */
+ * enqueued on the context stream without synchronising.  The site indices are
+ * not checked on the host: a query whose index lies outside 0..N-1 reads
+ * nothing and gets dE = NaN. */
 int sn_site_energy_batch_device(sn_ctx *ctx, int32_t replica, int64_t n, const int32_t *site_dev,
                                 const double *prop_dev, double *dE_dev);
 
@@ -164,6 +166,14 @@ int sn_mc_replay(sn_ctx *ctx, int32_t replica, int64_t n, const int32_t *site, c
 int sn_counters(sn_ctx *ctx, int32_t replica, uint64_t *accept, uint64_t *reject,
                 uint64_t *attempted);
 int sn_reset_counters(sn_ctx *ctx);
+
+/* Sweep index of replica r: the sweep counter that keys its counter-RNG draws
+ * (the next sweep draws as sweep `index` of the stream; 32 bits, it wraps).
+ * Setting it with the saved lattice resumes a run exactly where it stopped:
+ * the reference keeps only its seed and restarts its generator
+ * (src/starrynight-main.c:172-176), so it has no counterpart. */
+int sn_get_sweep_index(sn_ctx *ctx, int32_t replica, uint64_t *index);
+int sn_set_sweep_index(sn_ctx *ctx, int32_t replica, uint64_t index);
 
 /* ---- energy and adjoint (no reference counterpart: the reference has no
  * working total-energy function, src/starrynight-main.c:63) ---------------- */

@@ -23,7 +23,8 @@ EXPORTS = [
     "sn_set_lattice_device", "sn_gradient_device", "sn_gradient_all_device",
     "sn_get_lattice", "sn_lattice_device_ptr", "sn_init_lattice_random", "sn_set_params",
     "sn_site_energy_batch", "sn_site_energy_batch_device", "sn_mc_sweeps", "sn_sync",
-    "sn_mc_replay", "sn_counters", "sn_reset_counters", "sn_energy", "sn_gradient", "sn_moment",
+    "sn_mc_replay", "sn_counters", "sn_reset_counters", "sn_get_sweep_index", "sn_set_sweep_index",
+    "sn_energy", "sn_gradient", "sn_moment",
     "sn_radial_order", "sn_dipole_potential", "sn_dipole_efield", "sn_recombination",
     "sn_schedule_info", "sn_schedule_order", "sn_stream", "sn_sweep_config",
     "sn_time_sweep_kernel", "sn_plan_schedule", "sn_autotune", "sn_reference_initial_lattice",
@@ -86,6 +87,8 @@ def lib():
         L.sn_mc_replay.argtypes = [vp, C.c_int32, C.c_int64, ip, dp, dp, u8p, u8p, u8p, u64p, u64p]
         L.sn_counters.argtypes = [vp, C.c_int32, u64p, u64p, u64p]
         L.sn_reset_counters.argtypes = [vp]
+        L.sn_get_sweep_index.argtypes = [vp, C.c_int32, u64p]
+        L.sn_set_sweep_index.argtypes = [vp, C.c_int32, C.c_uint64]
         L.sn_energy.argtypes = [vp, C.c_int32, dp, dp]
         L.sn_gradient.argtypes = [vp, C.c_int32, dp, dp, dp]
         L.sn_gradient_device.argtypes = [vp, C.c_int32, vp, dp, dp]

@@ -14,7 +14,12 @@
  *   EfieldRamp = { axis = "z"; from = -0.1; to = 0.1; steps = 41; sweeps = 500; }
  *   Seed = n                        RNG key (default 0xDEADBEEF + T)
  *   Tile = [8, 8, 8]                tile edge the schedule aims for
- *   SaveLattice = "file"            final lattice of replica 0, double[N][4]
+ *   SaveLattice = "file"            final lattices, double[N][4] per replica (replica 0
+ *                                   first), and "file.sweep": one line "replica
+ *                                   sweep_index seed" per replica
+ *   LoadLattice = "file"            start from a SaveLattice file instead of the
+ *                                   initial pattern, each replica's draws resumed at
+ *                                   its saved sweep index (an exact restart)
  *   Autotune = true                 pick the sweep schedule by measurement (sn_autotune)
  *
  * CalculateRadialOrderParameter writes the reference's RDF files
@@ -215,6 +220,90 @@ static int load(const char *path, snc_config *cf, run_cfg *r)
     return 0;
 }
 
+/* SaveLattice: every replica's lattice, then "<file>.sweep" with each
+ * replica's sweep index and RNG key (the state a restart needs). */
+static int checkpoint(sn_ctx *ctx, const char *path, int R, long long N, const sn_params *ps, double *lat)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) { fprintf(stderr, "cannot write %s\n", path); return 1; }
+    for (int i = 0; i < R; i++) {
+        const int st = sn_get_lattice(ctx, i, lat);
+        if (st != SN_OK || fwrite(lat, sizeof(double) * 4, N, f) != (size_t)N) {
+            fprintf(stderr, "cannot write replica %d to %s (%s)\n", i, path, sn_status_string(st));
+            fclose(f);
+            return 1;
+        }
+    }
+    fclose(f);
+    char side[1024];
+    snprintf(side, sizeof(side), "%s.sweep", path);
+    FILE *g = fopen(side, "w");
+    if (!g) { fprintf(stderr, "cannot write %s\n", side); return 1; }
+    for (int i = 0; i < R; i++) {
+        uint64_t idx = 0;
+        const int st = sn_get_sweep_index(ctx, i, &idx);
+        if (st != SN_OK) {
+            fprintf(stderr, "sn_get_sweep_index(%d) failed: %s\n", i, sn_status_string(st));
+            fclose(g);
+            return 1;
+        }
+        fprintf(g, "%d %llu %llu\n", i, (unsigned long long)idx, (unsigned long long)ps[i].seed);
+    }
+    fclose(g);
+    return 0;
+}
+
+/* LoadLattice: the lattices and sweep indices of a SaveLattice file.  The
+ * replicas' RNG keys must match the saved ones, or the restart would not
+ * continue the same draw stream. */
+static int restore(sn_ctx *ctx, const char *path, int R, long long N, const sn_params *ps, double *lat)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) { fprintf(stderr, "cannot read %s\n", path); return 1; }
+    for (int i = 0; i < R; i++) {
+        if (fread(lat, sizeof(double) * 4, N, f) != (size_t)N) {
+            fprintf(stderr, "%s holds fewer than %d lattices of %lld sites\n", path, R, N);
+            fclose(f);
+            return 1;
+        }
+        const int st = sn_set_lattice(ctx, i, lat);
+        if (st != SN_OK) {
+            fprintf(stderr, "sn_set_lattice(%d) failed: %s (%s)\n", i, sn_status_string(st), sn_last_error(ctx));
+            fclose(f);
+            return 1;
+        }
+    }
+    fclose(f);
+    char side[1024];
+    snprintf(side, sizeof(side), "%s.sweep", path);
+    FILE *g = fopen(side, "r");
+    if (!g) { fprintf(stderr, "cannot read %s\n", side); return 1; }
+    for (int i = 0; i < R; i++) {
+        int r = -1;
+        unsigned long long idx = 0, seed = 0;
+        if (fscanf(g, "%d %llu %llu", &r, &idx, &seed) != 3 || r != i) {
+            fprintf(stderr, "%s: no sweep index for replica %d\n", side, i);
+            fclose(g);
+            return 1;
+        }
+        if (seed != (unsigned long long)ps[i].seed) {
+            fprintf(stderr, "%s: replica %d was saved with RNG key %llu, this run uses %llu\n", side, i, seed,
+                    (unsigned long long)ps[i].seed);
+            fclose(g);
+            return 1;
+        }
+        const int st = sn_set_sweep_index(ctx, i, idx);
+        if (st != SN_OK) {
+            fprintf(stderr, "sn_set_sweep_index(%d) failed: %s\n", i, sn_status_string(st));
+            fclose(g);
+            return 1;
+        }
+    }
+    fclose(g);
+    fprintf(stderr, "Restarted from %s\n", path);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     const char *path = "starrynight.cfg", *dry_run = NULL;
@@ -324,6 +413,8 @@ int main(int argc, char **argv)
         CHECK(ctx, sn_reference_initial_lattice(r.X, r.Y, r.Z, r.initial, mt_seed, r.nspec, r.len, r.prev, lat));
         CHECK(ctx, sn_set_lattice(ctx, i, lat));
     }
+    const char *load = NULL;
+    if (snc_lookup_string(&cf, "LoadLattice", &load) && restore(ctx, load, R, N, ps, lat)) return 1;
     /* the run log "Recombination_T_%04d.log" and its header (src/starrynight-main.c:164-178) */
     FILE **logs = (FILE **)calloc(R, sizeof(FILE *));
     if (!logs) return 1;
@@ -430,12 +521,7 @@ int main(int argc, char **argv)
     fprintf(stderr, "Monte Carlo moves - ACCEPT: %llu REJECT: %llu ratio: %f\n", (unsigned long long)A,
             (unsigned long long)Rj, (float)A / (float)(A + Rj));
     const char *save = NULL;
-    if (snc_lookup_string(&cf, "SaveLattice", &save)) {
-        CHECK(ctx, sn_get_lattice(ctx, 0, lat));
-        FILE *f = fopen(save, "wb");
-        if (!f || fwrite(lat, sizeof(double) * 4, N, f) != (size_t)N) { fprintf(stderr, "cannot write %s\n", save); return 1; }
-        fclose(f);
-    }
+    if (snc_lookup_string(&cf, "SaveLattice", &save) && checkpoint(ctx, save, R, N, ps, lat)) return 1;
     for (int i = 0; i < R; i++) fclose(logs[i]);
     free(logs);
     free(lat);

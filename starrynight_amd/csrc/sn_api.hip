@@ -1182,6 +1182,29 @@ int sn_reset_counters(sn_ctx *c)
     return SN_OK;
 }
 
+int sn_get_sweep_index(sn_ctx *c, int32_t r, uint64_t *index)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (!index) return fail(c, SN_ERR_INVALID, "null output");
+    unsigned v = 0;
+    HIPCHK(c, hipMemcpyAsync(&v, c->d_sweep + r, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *index = v;
+    return SN_OK;
+}
+
+int sn_set_sweep_index(sn_ctx *c, int32_t r, uint64_t index)
+{
+    int st = check_rep(c, r);
+    if (st) return st;
+    if (index > 0xFFFFFFFFull) return fail(c, SN_ERR_RANGE, "sweep index beyond 32 bits");
+    const unsigned v = (unsigned)index;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_sweep + r, &v, sizeof(v), hipMemcpyHostToDevice));
+    return SN_OK;
+}
+
 // Energy parts and parameter derivatives (kNPart sums per replica into
 // d_result) and the gradient into dg when non-null.  Lattices with a sweep
 // schedule and DipoleCutOff <= 5 take the tiled kernel (one workgroup per
@@ -1189,31 +1212,45 @@ int sn_reset_counters(sn_ctx *c)
 // r >= 0: replica r alone, dg holds its N x 3 rows.  r < 0: every replica,
 // dg holds all sites' rows in lattice order (replica r from row base_r), and
 // replica r's sums land at d_result + r * kNPart.
-static int run_energy(sn_ctx *c, int r, double *dg)
+static int run_energy(sn_ctx *c, int r, double *dg, double *fixed_partials = nullptr,
+                      size_t fixed_cap = 0)
 {
     static_assert(kEnergyParts == kNPart, "energy kernels share the partial-sum layout");
     const bool all = r < 0;
     int st;
+    // the partial sums: the context's growable buffer, or a caller-given fixed
+    // one of fixed_cap doubles (checked, never grown)
+    auto partials = [&](size_t n, double **out) -> int {
+        if (fixed_partials) {
+            if (n > fixed_cap)
+                return fail(c, SN_ERR_RANGE, "partial sums need " + std::to_string(n) + " doubles, the fixed scratch holds " +
+                                                 std::to_string(fixed_cap));
+            *out = fixed_partials;
+            return SN_OK;
+        }
+        const int e = ensure_partials(c, n);
+        *out = c->d_partials;
+        return e;
+    };
+    double *part = nullptr;
     if (c->energy_tiled) {
         const int t0 = all ? 0 : c->rep_tile_off[r], t1 = all ? c->rep_tile_off[c->R] : c->rep_tile_off[r + 1];
         const int nblk = t1 - t0;
-        if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
+        if ((st = partials((size_t)nblk * kNPart, &part))) return st;
         // one launch: the mixed-length variant serves uniform replicas too
         const bool mixed = all ? any_mixed(c) : c->reps[r].mixed != 0;
         EnergyArgs ea;
         ea.lat = c->d_lat; ea.reps = c->d_reps; ea.tiles = c->d_tiles; ea.shapes = c->d_shapes;
         ea.tile_list = c->d_tile_list + t0;
         ea.grad = dg; ea.grad_origin = all ? 0 : c->reps[r].base;
-        ea.partials = c->d_partials;
+        ea.partials = part;
         ea.region_doubles = field_region_doubles(c);
         const size_t lds = (size_t)(mixed ? 4 : 3) * ea.region_doubles * sizeof(double);
         hipLaunchKernelGGL(pick_energy(c->rc, c->two_d, mixed), dim3(nblk), dim3(kBlock), lds, c->stream, ea);
         HIPCHK(c, hipGetLastError());
-        if (all)
-            hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNPart + kReduceCols - 1) / kReduceCols, c->R),
-                               dim3(256), 0, c->stream, c->d_partials, nblk, kNPart, c->d_result, c->d_tile_seg);
-        else
-            launch_reduce(c, nblk, kNPart);
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNPart + kReduceCols - 1) / kReduceCols, all ? c->R : 1),
+                           dim3(256), 0, c->stream, part, nblk, kNPart, c->d_result,
+                           all ? (const int *)c->d_tile_seg : (const int *)nullptr);
         HIPCHK(c, hipGetLastError());
         return SN_OK;
     }
@@ -1222,13 +1259,13 @@ static int run_energy(sn_ctx *c, int r, double *dg)
     for (int q = all ? 0 : r; q < (all ? c->R : r + 1); q++) {
         const RepHost &h = c->reps[q];
         const int nblk = (int)((h.N + 255) / 256);
-        if ((st = ensure_partials(c, (size_t)nblk * kNPart))) return st;
+        if ((st = partials((size_t)nblk * kNPart, &part))) return st;
         double *g = dg ? dg + 3 * (all ? h.base : 0) : nullptr;
         hipLaunchKernelGGL(energy_grad_kernel, dim3(nblk), dim3(256), 0, c->stream, c->d_lat, c->d_reps, q,
-                           c->d_offs, c->noff, g, c->d_partials);
+                           c->d_offs, c->noff, g, part);
         HIPCHK(c, hipGetLastError());
         hipLaunchKernelGGL(reduce_partials_kernel, dim3((kNPart + kReduceCols - 1) / kReduceCols), dim3(256), 0,
-                           c->stream, c->d_partials, nblk, kNPart, c->d_result + (all ? q * kNPart : 0),
+                           c->stream, part, nblk, kNPart, c->d_result + (all ? q * kNPart : 0),
                            (const int *)nullptr);
         HIPCHK(c, hipGetLastError());
     }

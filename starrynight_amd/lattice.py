@@ -212,6 +212,17 @@ class StarryNight:
     def reset_counters(self):
         self._chk(lib().sn_reset_counters(self._h))
 
+    def sweep_index(self, replica: int = 0) -> int:
+        """The sweep counter keying replica `replica`'s counter-RNG draws."""
+        v = C.c_uint64(0)
+        self._chk(lib().sn_get_sweep_index(self._h, replica, C.byref(v)))
+        return v.value
+
+    def set_sweep_index(self, replica: int, index: int):
+        """Resume replica `replica`'s draw stream at sweep `index` (with its
+        saved lattice, a restarted run continues exactly)."""
+        self._chk(lib().sn_set_sweep_index(self._h, replica, int(index)))
+
     # -- energy / adjoint / observables
     def energy(self, replica: int = 0):
         e = C.c_double(0)

@@ -29,6 +29,12 @@ STATES = {
                                     lengths=(1.0, 0.5), prevalence=(0.5, 0.5)),
     "square25_T450": dict(X=25, Y=25, Z=1, rc=3, T=450.0, J=1.0, E=(0.0, 0.0, 0.0),
                           lengths=(1.0,), prevalence=(1.0,)),
+    # ordered phases: ferroelectric (CageStrain 4, <|P|> ~ 0.76) and the
+    # antiferroelectric order of CageStrain 1 at low temperature (E/N ~ -3.8)
+    "cube12_T800_J4_ferro": dict(X=12, Y=12, Z=12, rc=3, T=800.0, J=4.0, E=(0.0, 0.0, 0.0),
+                                 lengths=(1.0,), prevalence=(1.0,)),
+    "cube12_T150_antiferro": dict(X=12, Y=12, Z=12, rc=3, T=150.0, J=1.0, E=(0.0, 0.0, 0.0),
+                                  lengths=(1.0,), prevalence=(1.0,)),
 }
 
 

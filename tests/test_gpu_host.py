@@ -148,3 +148,42 @@ def test_host_autotune_with_species(sn, tmp_path, monkeypatch):
     out, err = run_host(tmp_path, cfg)
     assert "Autotune: tile" in err and err.count("MC Moves (per second):") == 3
     assert "|mixed|" in (tmp_path / "tune.txt").read_text()
+
+
+RESUME_CFG = """
+T: 300
+X = 16
+Y = 16
+Z = 16
+InitialLattice = "random"
+DipoleCutOff: 3
+CageStrain = 1.0
+MCEqmSteps: 0
+MCMegaSteps: {mega}
+MCMoves: 2.0
+TemperatureSchedule = [ 250, 350 ];
+{extra}
+"""
+
+
+def test_host_save_and_load_restart_exactly(sn, tmp_path):
+    """SaveLattice writes every replica's lattice and sweep index; a run that
+    starts from it with LoadLattice continues the same chains: 2 + 2
+    megasteps equal 4 megasteps bit for bit."""
+    print_case("sn_host_restart", cfg="16^3 two temperatures")
+    if not os.path.exists(HOST):
+        pytest.skip("sn_host not built")
+    a, b, c = tmp_path / "a", tmp_path / "b", tmp_path / "c"
+    for d in (a, b, c):
+        d.mkdir()
+    run_host(a, RESUME_CFG.format(mega=4, extra='SaveLattice = "straight.bin";'))
+    run_host(b, RESUME_CFG.format(mega=2, extra='SaveLattice = "half.bin";'))
+    (c / "half.bin").write_bytes((b / "half.bin").read_bytes())
+    (c / "half.bin.sweep").write_bytes((b / "half.bin.sweep").read_bytes())
+    _, err = run_host(c, RESUME_CFG.format(mega=2, extra='LoadLattice = "half.bin";\nSaveLattice = "resumed.bin";'))
+    assert "Restarted from half.bin" in err
+    straight = np.fromfile(a / "straight.bin", np.float64)
+    resumed = np.fromfile(c / "resumed.bin", np.float64)
+    assert straight.size == 2 * 16 ** 3 * 4 and np.array_equal(straight, resumed)
+    idx = [l.split() for l in (b / "half.bin.sweep").read_text().splitlines()]
+    assert [int(r[1]) for r in idx] == [4, 4]

@@ -305,3 +305,38 @@ def test_torus_greedy_batch(sn, oracle):
         o0, s0 = ctx.schedule(0)
         o2, s2 = ctx.schedule(2)
         assert np.array_equal(o0, o2) and np.array_equal(s0, s2)
+
+
+def test_resume_from_saved_lattice_and_sweep_index(sn):
+    """Exact restart: 3 + 3 sweeps across two contexts (the lattice and the
+    sweep index carried over) equal 6 sweeps in one, bit for bit, for every
+    replica of a batch."""
+    reps = [sn.ReplicaParams(16, 16, 16, T=300, seed=51), sn.ReplicaParams(20, 18, 16, T=250, seed=52)]
+    print_case("resume", n=2)
+    rng = np.random.default_rng(51)
+    lats = [rand_lattice(r.N, rng, mixed=True) for r in reps]
+    with sn.StarryNight(reps) as ctx:
+        for r, lat in enumerate(lats):
+            ctx.set_lattice(lat, r)
+        ctx.MC_sweeps(6)
+        straight = [ctx.get_lattice(r) for r in range(2)]
+        assert [ctx.sweep_index(r) for r in range(2)] == [6, 6]
+    with sn.StarryNight(reps) as ctx:
+        for r, lat in enumerate(lats):
+            ctx.set_lattice(lat, r)
+        ctx.MC_sweeps(3)
+        saved = [(ctx.get_lattice(r), ctx.sweep_index(r)) for r in range(2)]
+    with sn.StarryNight(reps) as ctx:
+        for r, (lat, idx) in enumerate(saved):
+            ctx.set_lattice(lat, r)
+            ctx.set_sweep_index(r, idx)
+        ctx.MC_sweeps(3)
+        resumed = [ctx.get_lattice(r) for r in range(2)]
+        assert [ctx.sweep_index(r) for r in range(2)] == [6, 6]
+    for a, b in zip(straight, resumed):
+        assert np.array_equal(a, b)
+    # without the index the restart draws the first sweeps' stream again
+    with sn.StarryNight(reps) as ctx:
+        ctx.set_lattice(saved[0][0], 0)
+        ctx.MC_sweeps(3)
+        assert not np.array_equal(ctx.get_lattice(0), straight[0])
