@@ -196,6 +196,20 @@ int sn_gradient_device(sn_ctx *ctx, int32_t replica, void *grad_dev, double dpar
  * dparams (double[R][6]) and energy (double[R]) may be NULL. */
 int sn_gradient_all_device(sn_ctx *ctx, void *grad_dev, double *dparams, double *energy);
 
+/* sn_gradient_all_device that allocates nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  It has no counterpart in the reference.  The per-
+ * workgroup partial sums live in a fixed device array of SN_NOALLOC_PARTIALS
+ * doubles (1 MiB) per GPU, held until the process ends and shared by every
+ * context on that GPU; the energy sums go through buffers sn_create already
+ * allocated.  It needs 10 doubles per tile of all replicas (or per 256 sites
+ * of the largest replica when the lattices have no tiling) and returns
+ * SN_ERR_RANGE beyond the array.  Calls that overlap on one GPU (contexts used
+ * from several host threads) race on the array.  Outputs as
+ * sn_gradient_all_device. */
+#define SN_NOALLOC_PARTIALS 131072
+int sn_gradient_all_device_noalloc(sn_ctx *ctx, void *grad_dev, double *dparams, double *energy);
+
 /* Sum of dipole vectors of replica r (M = sum_i p_i, vacancies included as the
  * reference's landau_order / polarisation do, src/starrynight-analysis.c:48-62,
  * 506-526). */

@@ -20,7 +20,7 @@ STATUS = {0: "SN_OK", 1: "SN_ERR_INVALID", 2: "SN_ERR_HIP", 3: "SN_ERR_NOMEM", 4
 # every entry point declared in include/starrynight_amd.h
 EXPORTS = [
     "sn_create", "sn_destroy", "sn_last_error", "sn_status_string", "sn_set_lattice",
-    "sn_set_lattice_device", "sn_gradient_device", "sn_gradient_all_device",
+    "sn_set_lattice_device", "sn_gradient_device", "sn_gradient_all_device", "sn_gradient_all_device_noalloc",
     "sn_get_lattice", "sn_lattice_device_ptr", "sn_init_lattice_random", "sn_set_params",
     "sn_site_energy_batch", "sn_site_energy_batch_device", "sn_mc_sweeps", "sn_sync",
     "sn_mc_replay", "sn_counters", "sn_reset_counters", "sn_get_sweep_index", "sn_set_sweep_index",
@@ -93,6 +93,7 @@ def lib():
         L.sn_gradient.argtypes = [vp, C.c_int32, dp, dp, dp]
         L.sn_gradient_device.argtypes = [vp, C.c_int32, vp, dp, dp]
         L.sn_gradient_all_device.argtypes = [vp, vp, dp, dp]
+        L.sn_gradient_all_device_noalloc.argtypes = [vp, vp, dp, dp]
         L.sn_set_lattice_device.argtypes = [vp, C.c_int32, vp]
         L.sn_moment.argtypes = [vp, C.c_int32, dp]
         L.sn_radial_order.argtypes = [vp, C.c_int32, dp, dp, C.POINTER(C.c_int64)]

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,16 @@ struct RepHost {
 };
 
 }  // namespace
+
+// Partial sums of sn_gradient_all_device_noalloc: a fixed device array, one
+// copy per GPU, held until the process ends.  Host code never uses the name
+// `scratch` as an address (it is a host-side stand-in): its device address is
+// resolved once per device with hipGetSymbolAddress (resolve_scratch, called
+// from sn_create with the context's device current) and kept below.
+static __device__ __align__(256) double scratch[SN_NOALLOC_PARTIALS];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_dev[kMaxDevices];   // device address of `scratch` per device id
+static std::mutex g_scratch_mu;
 
 struct sn_ctx {
     int R = 0, rc = 0;
@@ -70,6 +81,7 @@ struct sn_ctx {
     // copies, species tables): grows as needed, freed with the context
     void *d_stage = nullptr;
     size_t stage_cap = 0;
+    double *d_scratch = nullptr;   // this device's `scratch` array (not owned)
     int max_tile_sites = 0, max_cls = 0, max_chunks = 0;
     int field_region = 0;   // largest staged region of the field kernel (z rows padded to odd)
     int *d_tile_list = nullptr;          // tile indices grouped by replica (tiled energy kernel)
@@ -456,6 +468,23 @@ static int setup_torus(sn_ctx *c)
     return SN_OK;
 }
 
+// This device's address of `scratch`, resolved on the first context created
+// on it (the device is current), then reused by every later context.
+static int resolve_scratch(sn_ctx *c)
+{
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, SN_ERR_UNSUPPORTED, "device id beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(c, SN_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[c->device] = p;
+    }
+    c->d_scratch = (double *)g_scratch_dev[c->device];
+    return SN_OK;
+}
+
 int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuning, sn_ctx **out)
 {
     if (!out) return SN_ERR_INVALID;
@@ -550,6 +579,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
 
     HIPCHK(c, hipGetDevice(&c->device));
+    if (int se = resolve_scratch(c)) return se;
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(c, hipMalloc((void **)&c->d_lat, (size_t)c->total_sites * sizeof(double4)));
     HIPCHK(c, hipMemset(c->d_lat, 0, (size_t)c->total_sites * sizeof(double4)));
@@ -1326,6 +1356,25 @@ int sn_gradient_all_device(sn_ctx *c, void *grad_dev, double *dparams, double *e
     if (!c) return SN_ERR_INVALID;
     int st;
     if ((st = run_energy(c, -1, (double *)grad_dev)) || (st = fetch_result(c, (size_t)c->R * kNPart))) return st;
+    for (int r = 0; r < c->R; r++) {
+        const double *res = c->h_result + (size_t)r * kNPart;
+        if (dparams) for (int i = 0; i < 6; i++) dparams[6 * r + i] = res[4 + i];
+        if (energy) energy[r] = energy_total(res);
+    }
+    return SN_OK;
+}
+
+int sn_gradient_all_device_noalloc(sn_ctx *c, void *grad_dev, double *dparams, double *energy)
+{
+    if (!c) return SN_ERR_INVALID;
+    if (!c->d_scratch) return fail(c, SN_ERR_HIP, "no-allocation scratch not resolved");
+    // the partials go to this device's fixed `scratch` array (every row the
+    // reduction reads is written first, so nothing needs clearing); the sums
+    // land in d_result and the pinned h_result, both allocated by sn_create
+    int st;
+    if ((st = run_energy(c, -1, (double *)grad_dev, c->d_scratch, SN_NOALLOC_PARTIALS)) ||
+        (st = fetch_result(c, (size_t)c->R * kNPart)))
+        return st;
     for (int r = 0; r < c->R; r++) {
         const double *res = c->h_result + (size_t)r * kNPart;
         if (dparams) for (int i = 0; i < 6; i++) dparams[6 * r + i] = res[4 + i];
