@@ -27,12 +27,12 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from tests.stat_observables import (RDF_BINS, STATES, beta_of, block_stats,  # noqa: E402
-                                    random_lattice, sample_scalars, species_lengths)
+                                    initial_lattice, sample_scalars, species_lengths)
 
 SCALAR_EVERY = 2
 RDF_EVERY = 20
 BURN = 2000
-NBLOCKS = 25
+NBLOCKS = 5   # blocks of 3600 sweeps per chain
 
 
 def run_chain(args):
@@ -44,7 +44,7 @@ def run_chain(args):
     beta = beta_of(st["T"])
     o = O.Oracle(X, Y, Z, st["rc"], J=st["J"], E=st["E"], K=0.0, constrain_to_x=False, beta=beta)
     L = species_lengths(N, st["lengths"], st["prevalence"], seed=7)
-    lat = random_lattice(N, L, seed=1000 + chain)
+    lat = initial_lattice(st, N, L, seed=1000 + chain)
     mt = O.MT19937(0xC0FFEE + 7919 * chain)
     o.mc_moves_f64(lat, mt, BURN * N)
     sc, rd = [], []

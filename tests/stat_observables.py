@@ -13,7 +13,12 @@ Derived: chi = beta N (<m2> - <m>^2) over all samples (finite-lattice
 susceptibility of the order-parameter magnitude).  Errors come from block
 means: every chain is cut into equal blocks much longer than the
 autocorrelation time; the standard error of a mean is the spread of the block
-means over sqrt(#blocks), that of chi a leave-one-block-out jackknife.
+means over sqrt(#blocks), that of chi a leave-one-block-out jackknife.  The
+CPU chains run 5 blocks of 3600 sweeps each; every GPU replica is one block
+of 2000 sweeps (after 2000 of burn-in), so its error is the spread over
+independent chains.  The slowest mode is the tilt of the whole polarisation
+in the ordered ferroelectric state (~10^2 sweeps); blocks of 80 sweeps
+underestimated the error of mz there by about a factor of two.
 """
 from __future__ import annotations
 
@@ -29,11 +34,24 @@ STATES = {
                                     lengths=(1.0, 0.5), prevalence=(0.5, 0.5)),
     "square25_T450": dict(X=25, Y=25, Z=1, rc=3, T=450.0, J=1.0, E=(0.0, 0.0, 0.0),
                           lengths=(1.0,), prevalence=(1.0,)),
-    # ordered phases: ferroelectric (CageStrain 4, <|P|> ~ 0.76) and the
-    # antiferroelectric order of CageStrain 1 at low temperature (E/N ~ -3.8)
-    "cube12_T800_J4_ferro": dict(X=12, Y=12, Z=12, rc=3, T=800.0, J=4.0, E=(0.0, 0.0, 0.0),
-                                 lengths=(1.0,), prevalence=(1.0,)),
-    "cube12_T150_antiferro": dict(X=12, Y=12, Z=12, rc=3, T=150.0, J=1.0, E=(0.0, 0.0, 0.0),
+    # ordered phases: ferroelectric (CageStrain 4, <|P|> ~ 0.79) and the
+    # antiferroelectric order of CageStrain 1 at low temperature (E/N ~ -3.65,
+    # fe[r2=12] ~ -0.73).  At 200 K random starts order within ~2000 sweeps;
+    # at 150 K (acceptance 0.06) chains keep distinct long-lived defect states
+    # for thousands of sweeps, so a handful of chains does not sample the
+    # distribution.
+    # The ferroelectric chains start polarised along +x in a field along -x
+    # (the field term of site_energy is +E.p, src/starrynight-montecarlo-core.c:120-121,
+    # so dipoles line up against E).  From random starts some chains stay in
+    # multi-domain states for >10^4 sweeps at 12^3; without the field the
+    # direction of the polarisation wanders over the sphere more slowly than
+    # the chains run, so averages of direction-dependent observables (mz, afe)
+    # over a handful of chains depend on where each chain happened to drift.
+    # The field (0.2: about 130 theta^2 of energy for a tilt theta of the
+    # whole lattice, against kT = 2.67) holds the direction near +x.
+    "cube12_T800_J4_ferro_Ex": dict(X=12, Y=12, Z=12, rc=3, T=800.0, J=4.0, E=(-0.2, 0.0, 0.0),
+                                    lengths=(1.0,), prevalence=(1.0,), initial="ferro_x"),
+    "cube12_T200_antiferro": dict(X=12, Y=12, Z=12, rc=3, T=200.0, J=1.0, E=(0.0, 0.0, 0.0),
                                   lengths=(1.0,), prevalence=(1.0,)),
 }
 
@@ -56,6 +74,17 @@ def random_lattice(N: int, L: np.ndarray, seed: int) -> np.ndarray:
     v = rng.normal(size=(N, 3))
     v /= np.linalg.norm(v, axis=1)[:, None]
     return np.concatenate([v, L[:, None]], 1)
+
+
+def initial_lattice(st: dict, N: int, L: np.ndarray, seed: int) -> np.ndarray:
+    """The chains' starting lattice: random unit dipoles, or all along +x
+    for a state point marked initial="ferro_x"."""
+    if st.get("initial") == "ferro_x":
+        lat = np.zeros((N, 4))
+        lat[:, 0] = 1.0
+        lat[:, 3] = L
+        return lat
+    return random_lattice(N, L, seed)
 
 
 def sample_scalars(energy: float, M: np.ndarray, N: int) -> np.ndarray:

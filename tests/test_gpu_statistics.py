@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 
 from tests.helpers import print_case
-from tests.stat_observables import (RDF_BINS, STATES, beta_of, block_stats, random_lattice,
+from tests.stat_observables import (RDF_BINS, STATES, beta_of, block_stats, initial_lattice,
                                     sample_scalars, species_lengths)
 
 pytestmark = pytest.mark.gpu
@@ -33,7 +33,7 @@ BURN = 2000
 SWEEPS = 2000
 SCALAR_EVERY = 2
 RDF_EVERY = 20
-NBLOCKS = 25
+NBLOCKS = 1   # each replica is one block (independent chains)
 ALPHA = 0.01   # family-wise error rate of one state point's z-tests
 
 
@@ -71,7 +71,7 @@ def test_equilibrium_averages_match_cpu_chains(sn, name):
     rd = [[] for _ in range(REPLICAS)]
     with sn.StarryNight(reps) as ctx:
         for r in range(REPLICAS):
-            ctx.set_lattice(random_lattice(N, L, seed=5000 + r), r)
+            ctx.set_lattice(initial_lattice(st, N, L, seed=5000 + r), r)
         ctx.MC_sweeps(BURN)
         for s in range(SCALAR_EVERY, SWEEPS + 1, SCALAR_EVERY):
             ctx.MC_sweeps(SCALAR_EVERY)
