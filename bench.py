@@ -256,7 +256,7 @@ def main():
     mean_ms, nl = ctx.time_sweep_kernel(ksweeps)
     updates_per_launch = N * ksweeps / nl
     achieved = updates_per_launch * F / (mean_ms * 1e-3) / 1e12
-    field = (tun is None or tun.kernel != 2) and 1 <= rc <= 5
+    field = tun is not None and tun.kernel == 1
     kname = "field_sweep_kernel" if field else "tile_sweep_kernel"
     traffic = measured_traffic(kname, L, rc)
     hbm_alg = updates_per_launch * BYTES_PER_UPDATE / (mean_ms * 1e-3) / 1e9

@@ -239,6 +239,9 @@ static void launch_reduce(sn_ctx *c, int nblk, int width)
 // Largest tile + halo (sites) staged in LDS: 4 doubles per site (mixed lengths)
 // plus the proposal slots and shape tables stay under the 160 KB of a CU.
 constexpr int kMaxRegion = 4096;
+// Replicas of at most this many sites sweep with the whole-lattice kernel
+// when the tuning leaves the kernel to the library.
+constexpr long long kAutoTorusSites = 2048;
 // Largest tile: the field kernel gives each lane at most 2 items of 2 sites
 // (3D) or 1 site (2D).
 constexpr int kMaxTile3D = kFieldMaxTile3D, kMaxTile2D = kFieldMaxTile2D;
@@ -562,7 +565,14 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     const int blk = c->tuning.block_threads > 0 ? c->tuning.block_threads : kBlock;
     if (blk != 256 && blk != 512) return fail(c, SN_ERR_INVALID, "block_threads must be 256 or 512");
     if (c->tuning.kernel < 0 || c->tuning.kernel > 3) return fail(c, SN_ERR_INVALID, "kernel must be 0..3");
-    if (blk != kBlock && c->tuning.kernel != 2)
+    // kernel 0 picks by size (rates measured on MI355X): the whole-lattice
+    // kernel for replicas of up to kAutoTorusSites sites (1.6-3x the gather
+    // kernel at 12^3 and 25x25), else the per-phase gather kernel (1.2-2x the
+    // field-update kernel from 16^3 to 128^3 at DipoleCutOff 2-4, batched or
+    // not), else the field-update kernel where no gather instance fits
+    int kern = c->tuning.kernel;
+    if (kern == 0 && blk != kBlock) kern = 2;
+    if (blk != kBlock && kern != 2)
         return fail(c, SN_ERR_INVALID, "block_threads applies to the gather kernel (kernel = 2) only");
     c->blk = blk;
     int G = c->tuning.lanes_per_site;
@@ -575,7 +585,15 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     while (NPL < need) NPL <<= 1;
     c->G = G; c->NPL = NPL;
     const bool field_ok = pick_field(c->rc, c->two_d, false) != nullptr;
-    if (c->can_sweep && (c->tuning.kernel == 2 || !field_ok) && !pick_sweep(G, NPL, false, blk))
+    const bool gather_ok = pick_sweep(G, NPL, false, blk) != nullptr;
+    bool auto_torus = false;
+    if (kern == 0) {
+        long long largest = 0;
+        for (const RepHost &h : c->reps) largest = std::max(largest, h.N);
+        auto_torus = largest <= kAutoTorusSites;
+        kern = (gather_ok || !field_ok) ? 2 : 1;
+    }
+    if (c->can_sweep && (kern == 2 || !field_ok) && !gather_ok)
         return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
 
     HIPCHK(c, hipGetDevice(&c->device));
@@ -692,8 +710,8 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     c->replay_counts.assign(2 * c->R, 0ull);
 
     // kernel choice: the field-update kernel where it is instantiated
-    c->use_field = c->tuning.kernel != 2 && pick_field(c->rc, c->two_d, false) != nullptr;
-    if (c->tuning.kernel == 1 && !c->use_field)
+    c->use_field = kern != 2 && field_ok;
+    if (kern == 1 && !c->use_field)
         return fail(c, SN_ERR_UNSUPPORTED, "field-update kernel needs DipoleCutOff in 1..5");
     size_t need_lds = c->use_field ? std::max(field_lds(c, false), field_lds(c, true))
                                    : sweep_lds(c, true);
@@ -703,9 +721,16 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         c->sweep_err = "tile + halo does not fit in LDS; use a smaller tile";
     }
     const bool tiles_ok = c->can_sweep;
-    if (c->tuning.kernel == 3) {
+    if (kern == 3) {
         if ((st = setup_torus(c))) return st;
         c->can_sweep = true;
+    } else if (auto_torus) {
+        // lattices the whole-lattice kernel cannot colour keep the tiles
+        const std::string keep = c->err;
+        const int ts = setup_torus(c);
+        if (ts == SN_OK) c->can_sweep = true;
+        else if (ts != SN_ERR_UNSUPPORTED) return ts;
+        else c->err = keep;
     }
     if (!tiles_ok) return SN_OK;
     // the tiled energy kernel stages the field kernel's padded region

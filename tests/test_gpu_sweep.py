@@ -13,23 +13,27 @@ from tests.helpers import oracle_model, print_case, rand_lattice
 pytestmark = pytest.mark.gpu
 
 CASES = {
-    "16cube_rc3_uniform": (dict(X=16, Y=16, Z=16, T=300), False, {}),
-    "16cube_rc3_mixed_field_K": (dict(X=16, Y=16, Z=16, T=250, Efield=(0.05, 0.0, 0.1), K=0.3), True, {}),
-    "25x25x1_rc3": (dict(X=25, Y=25, Z=1, T=300), False, {}),
-    "20cube_rc4_mixed": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300), True, {}),
-    "12x16x20_rc3": (dict(X=12, Y=16, Z=20, T=400, CageStrain=2.0), False, {}),
-    "16cube_constrain_x": (dict(X=16, Y=16, Z=16, T=300, ConstrainToX=True), False, {}),
-    "16cube_T0": (dict(X=16, Y=16, Z=16, T=0), False, {}),
-    "16cube_lanes8_minidx": (dict(X=16, Y=16, Z=16, T=300), True, dict(lanes_per_site=8, fine_colouring=2)),
-    "16cube_lanes32_cubic": (dict(X=16, Y=16, Z=16, T=300), False, dict(lanes_per_site=32, fine_colouring=1)),
-    "18cube_direct_launch": (dict(X=18, Y=18, Z=18, T=300), False, dict(use_graph=-1)),
+    # the library's own choice (kernel 0): gather tiles at 16^3, whole-lattice at 12^3 and 25x25
+    "16cube_auto": (dict(X=16, Y=16, Z=16, T=300, Efield=(0.0, 0.02, 0.0)), True, {}),
+    "12cube_auto": (dict(X=12, Y=12, Z=12, T=300), False, {}),
+    "25x25x1_auto": (dict(X=25, Y=25, Z=1, T=300, K=0.1), True, {}),
+    "16cube_rc3_uniform": (dict(X=16, Y=16, Z=16, T=300), False, dict(kernel=1)),
+    "16cube_rc3_mixed_field_K": (dict(X=16, Y=16, Z=16, T=250, Efield=(0.05, 0.0, 0.1), K=0.3), True, dict(kernel=1)),
+    "25x25x1_rc3": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=1)),
+    "20cube_rc4_mixed": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300), True, dict(kernel=1)),
+    "12x16x20_rc3": (dict(X=12, Y=16, Z=20, T=400, CageStrain=2.0), False, dict(kernel=1)),
+    "16cube_constrain_x": (dict(X=16, Y=16, Z=16, T=300, ConstrainToX=True), False, dict(kernel=1)),
+    "16cube_T0": (dict(X=16, Y=16, Z=16, T=0), False, dict(kernel=1)),
+    "16cube_lanes8_minidx": (dict(X=16, Y=16, Z=16, T=300), True, dict(kernel=1, lanes_per_site=8, fine_colouring=2)),
+    "16cube_lanes32_cubic": (dict(X=16, Y=16, Z=16, T=300), False, dict(kernel=1, lanes_per_site=32, fine_colouring=1)),
+    "18cube_direct_launch": (dict(X=18, Y=18, Z=18, T=300), False, dict(kernel=1, use_graph=-1)),
     "16cube_gather_kernel": (dict(X=16, Y=16, Z=16, T=300, Efield=(0, 0.05, 0)), True, dict(kernel=2)),
     "25x25x1_gather_kernel": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=2)),
     "20cube_rc4_gather_kernel": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300), True, dict(kernel=2)),
-    "odd_tiles_17x19x21": (dict(X=17, Y=19, Z=21, T=350), True, {}),
-    "2d_mixed_30x26": (dict(X=30, Y=26, Z=1, T=200, K=0.2), True, {}),
-    "rc2_12cube": (dict(X=12, Y=12, Z=12, DipoleCutOff=2, T=300), True, {}),
-    "rc5_24cube": (dict(X=24, Y=24, Z=24, DipoleCutOff=5, T=300), False, {}),
+    "odd_tiles_17x19x21": (dict(X=17, Y=19, Z=21, T=350), True, dict(kernel=1)),
+    "2d_mixed_30x26": (dict(X=30, Y=26, Z=1, T=200, K=0.2), True, dict(kernel=1)),
+    "rc2_12cube": (dict(X=12, Y=12, Z=12, DipoleCutOff=2, T=300), True, dict(kernel=1)),
+    "rc5_24cube": (dict(X=24, Y=24, Z=24, DipoleCutOff=5, T=300), False, dict(kernel=1)),
     "16cube_gather_512": (dict(X=16, Y=16, Z=16, T=300, K=0.2), True, dict(kernel=2, block_threads=512)),
     "20cube_rc4_gather_512": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300), True,
                               dict(kernel=2, block_threads=512)),
@@ -210,7 +214,7 @@ def test_device_random_init(sn):
 def test_untileable_lattice_refuses_sweeps_but_gathers_work(sn, oracle):
     rp = sn.ReplicaParams(9, 9, 9, T=300)
     lat = rand_lattice(rp.N, np.random.default_rng(9))
-    with sn.StarryNight(rp) as ctx:
+    with sn.StarryNight(rp, sn.TuningParams(kernel=1)) as ctx:
         ctx.set_lattice(lat)
         with pytest.raises(sn.SnError) as e:
             ctx.MC_sweeps(1)
@@ -218,6 +222,24 @@ def test_untileable_lattice_refuses_sweeps_but_gathers_work(sn, oracle):
         dE = ctx.site_energy_batch(np.arange(10, dtype=np.int32), lat[10:20, :3])
     want, _ = oracle_model(oracle, rp).site_energy_batch(lat, np.arange(10), lat[10:20, :3])
     assert np.abs(dE - want).max() < 1e-12 * 100
+
+
+def test_default_kernel_choice(sn):
+    """Kernel 0 (and no tuning at all): whole-lattice sweeps for replicas of
+    up to 2048 sites, tiles with the gather kernel above that.  9^3 has no
+    tiling at rc = 3 (tile kernels refuse it) but a periodic colouring."""
+    cases = [([sn.ReplicaParams(12, 12, 12, T=300)], True),
+             ([sn.ReplicaParams(9, 9, 9, T=300)], True),
+             ([sn.ReplicaParams(25, 25, 1, T=300)], True),
+             ([sn.ReplicaParams(16, 16, 16, T=300)], False),
+             ([sn.ReplicaParams(12, 12, 12, T=300), sn.ReplicaParams(16, 16, 16, T=300)], False)]
+    for reps, whole in cases:
+        for tun in (None, sn.TuningParams()):
+            with sn.StarryNight(reps, tun) as ctx:
+                cfg = ctx.sweep_config()
+                assert (cfg["tiles"] == 0) == whole, (reps[0].X, cfg)
+                if not whole:
+                    assert cfg["lanes_per_site"] == 16
 
 
 def test_field_and_gather_kernels_agree(sn):
