@@ -229,6 +229,17 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     const double wn0 = (!MIXED && nnf != 0.0 && rep.L0 != 0.0) ? w[0] - J / (rep.L0 * rep.L0) : w[0];
     unsigned nacc = 0, nrej = 0;
     __syncthreads();
+    if (MIXED) {
+        // vacant sites (length 0) leave the tile's step table: the reference
+        // returns from MC_move for them before drawing or counting
+        // (src/starrynight-montecarlo-core.c:163), so their groups skip the
+        // step (measured on 128^3 DipoleCutOff 4 half vacant: 6-21% faster)
+        for (int i = tid; i < s_gchunk_total(sh) * SW; i += BLK) {
+            const int e_ = s_gstep[i];
+            if (i % SW != 0 && e_ >= 0 && sl[e_ & 0xFFFF] == 0.0) s_gstep[i] = -1;
+        }
+        __syncthreads();
+    }
 
     for (int ch = 0; ch < sh.nchunks; ch++) {
         const int c0 = s_chunk[ch], c1 = s_chunk[ch + 1];
