@@ -444,36 +444,74 @@ int main(int argc, char **argv)
     if (want_efield)   /* "equilib_lattice_efield.xyz" (src/starrynight-main.c:225) */
         for (int i = 0; i < R; i++) CHECK(ctx, write_maps(ctx, i, R, "equilib_lattice", r.X, r.Y, r.Z, 0, 0, 1, lat));
 
-    /* optional field ramp (hysteresis): steps of `sweeps_per` sweeps each */
+    /* optional field ramp (hysteresis): `steps` field values from `from` to
+     * `to`, `sweeps` sweeps at each; loop = true runs back from `to` to
+     * `from` as well (the P(E) loop of figures/hysterisis/hysterisis.gpt).
+     * Each step prints the mean over replicas of P = sum_i p_i / N along the
+     * axis and its standard error over replicas: with sample = k > 0, P is
+     * averaged over the step's second half, read every k sweeps; with
+     * sample = 0, P is read once at the end of the step. */
     const snc_node *ramp = snc_lookup(&cf, "EfieldRamp");
     if (ramp) {
         const char *axis = "z";
         double from = -0.1, to = 0.1;
-        int steps = 41, per = 500;
+        int steps = 41, per = 500, loop = 0, sample = 0;
         snc_lookup_string(&cf, "EfieldRamp.axis", &axis);
         snc_lookup_float(&cf, "EfieldRamp.from", &from);
         snc_lookup_float(&cf, "EfieldRamp.to", &to);
         snc_lookup_int(&cf, "EfieldRamp.steps", &steps);
         snc_lookup_int(&cf, "EfieldRamp.sweeps", &per);
-        int ax = axis[0] == 'x' ? 0 : axis[0] == 'y' ? 1 : 2;
-        printf("# step E_%c  mean_P_%c_over_replicas\n", "xyz"[ax], "xyz"[ax]);
-        for (int s = 0; s < steps; s++) {
-            double Ev = steps > 1 ? from + (to - from) * s / (steps - 1) : from;
-            for (int i = 0; i < R; i++) {
-                double E[3] = {r.E[0], r.E[1], r.E[2]};
-                E[ax] = Ev;
-                CHECK(ctx, sn_set_params(ctx, i, Ts[i % nT], E, r.J));
-            }
-            CHECK(ctx, sn_mc_sweeps(ctx, per));
-            double Pm = 0.0;
-            for (int i = 0; i < R; i++) {
-                double M[3];
-                CHECK(ctx, sn_moment(ctx, i, M));
-                Pm += M[ax] / (double)N;
-            }
-            printf("%d %f %f\n", s, Ev, Pm / R);
-            fflush(stdout);
+        snc_lookup_bool(&cf, "EfieldRamp.loop", &loop);
+        snc_lookup_int(&cf, "EfieldRamp.sample", &sample);
+        if (steps < 1 || per < 1 || sample < 0 || (sample > 0 && sample > per / 2)) {
+            fprintf(stderr, "EfieldRamp: steps >= 1, sweeps >= 1 and 0 <= sample <= sweeps / 2\n");
+            return 1;
         }
+        int ax = axis[0] == 'x' ? 0 : axis[0] == 'y' ? 1 : 2;
+        double *acc = (double *)calloc((size_t)R, sizeof(double));
+        printf("# step branch E_%c mean_P_%c_over_replicas stderr_over_replicas\n", "xyz"[ax], "xyz"[ax]);
+        for (int b = 0; b < (loop ? 2 : 1); b++) {
+            for (int k = 0; k < steps; k++) {
+                const int s = b == 0 ? k : steps - 1 - k;
+                double Ev = steps > 1 ? from + (to - from) * s / (steps - 1) : from;
+                for (int i = 0; i < R; i++) {
+                    double E[3] = {r.E[0], r.E[1], r.E[2]};
+                    E[ax] = Ev;
+                    CHECK(ctx, sn_set_params(ctx, i, Ts[i % nT], E, r.J));
+                }
+                int nsamp = 0;
+                for (int i = 0; i < R; i++) acc[i] = 0.0;
+                if (sample > 0) {
+                    const int first = per - (per / 2 / sample) * sample;   /* the second half, in reads of `sample` */
+                    CHECK(ctx, sn_mc_sweeps(ctx, first));
+                    for (int d = first; d < per; d += sample) {
+                        CHECK(ctx, sn_mc_sweeps(ctx, sample));
+                        for (int i = 0; i < R; i++) {
+                            double M[3];
+                            CHECK(ctx, sn_moment(ctx, i, M));
+                            acc[i] += M[ax] / (double)N;
+                        }
+                        nsamp++;
+                    }
+                } else {
+                    CHECK(ctx, sn_mc_sweeps(ctx, per));
+                    for (int i = 0; i < R; i++) {
+                        double M[3];
+                        CHECK(ctx, sn_moment(ctx, i, M));
+                        acc[i] = M[ax] / (double)N;
+                    }
+                    nsamp = 1;
+                }
+                double m = 0.0, v = 0.0;
+                for (int i = 0; i < R; i++) m += acc[i] / nsamp;
+                m /= R;
+                for (int i = 0; i < R; i++) v += (acc[i] / nsamp - m) * (acc[i] / nsamp - m);
+                const double se = R > 1 ? sqrt(v / (R - 1) / R) : 0.0;
+                printf("%d %s %f %f %f\n", s, b == 0 ? "up" : "down", Ev, m, se);
+                fflush(stdout);
+            }
+        }
+        free(acc);
     }
 
     /* production: MCMegaSteps x MC_moves, timed around the moves only */

@@ -187,3 +187,47 @@ def test_host_save_and_load_restart_exactly(sn, tmp_path):
     assert straight.size == 2 * 16 ** 3 * 4 and np.array_equal(straight, resumed)
     idx = [l.split() for l in (b / "half.bin.sweep").read_text().splitlines()]
     assert [int(r[1]) for r in idx] == [4, 4]
+
+
+RAMP_CFG = """
+T: 300
+X = 12
+Y = 12
+Z = 12
+InitialLattice = "random"
+DipoleCutOff: 4
+Dipoles    = [ 1.0, 0.0 ];
+Prevalence = [ 0.5, 0.5 ];
+CageStrain = 1.0
+MCEqmSteps: 0
+MCMegaSteps: 0
+MCMoves: 1.0
+Replicas = 4;
+EfieldRamp = { axis = "z"; from = -0.1; to = 0.1; steps = 5; sweeps = 20; {extra} };
+"""
+
+
+def test_host_field_ramp_loop(sn, tmp_path):
+    """EfieldRamp with loop = true runs the field up and back down (the P(E)
+    loop); with sample = k the step's P is the mean of reads every k sweeps
+    over its second half, and each row carries the standard error over
+    replicas."""
+    print_case("sn_host_ramp", cfg="12^3 rc=4 solid solution, 4 replicas")
+    if not os.path.exists(HOST):
+        pytest.skip("sn_host not built")
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    out, _ = run_host(a, RAMP_CFG.replace("{extra}", "loop = true; sample = 2;"))
+    rows = [l.split() for l in out.splitlines() if l and not l.startswith("#")]
+    assert [r[1] for r in rows] == ["up"] * 5 + ["down"] * 5
+    E = [float(r[2]) for r in rows]
+    assert np.allclose(E, [-0.1, -0.05, 0.0, 0.05, 0.1, 0.1, 0.05, 0.0, -0.05, -0.1])
+    assert [int(r[0]) for r in rows] == [0, 1, 2, 3, 4, 4, 3, 2, 1, 0]
+    P, se = np.array([float(r[3]) for r in rows]), np.array([float(r[4]) for r in rows])
+    assert np.all(np.abs(P) <= 1.0) and np.all(se >= 0.0) and np.any(se > 0.0)
+    # one branch (the default), P read once at the end of each step
+    out1, _ = run_host(b, RAMP_CFG.replace("{extra}", "loop = false;"))
+    rows1 = [l.split() for l in out1.splitlines() if l and not l.startswith("#")]
+    assert [r[1] for r in rows1] == ["up"] * 5
+    assert np.allclose([float(r[2]) for r in rows1], E[:5])
