@@ -183,8 +183,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the built-in schedule instead of the autotuner's pick")
     ap.add_argument("--schedule", default=None,
-                    help="fixed schedule 'tx,ty,tz,colouring,kernel' instead of autotuning "
-                         "(profiler runs: keeps trial launches out of the trace)")
+                    help="fixed schedule 'tx,ty,tz,colouring,kernel[,threads]' instead of autotuning "
+                         "(profiler runs: keeps trial launches out of the trace); with --workload it "
+                         "applies to that workload's sweeps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the headline sweeps (for profiler runs)")
@@ -201,8 +202,8 @@ def main():
         copy_kernel()
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(replicas and slabs would shard across GPUs, see the design notes).", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (replicas, and slabs of "
+              "one large lattice, would shard one per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import torch  # noqa: F401  (loaded before the library: one HIP runtime for both)
@@ -213,7 +214,8 @@ def main():
         print(json.dumps(run_batched(sn, args.rc, log, autotune=not args.no_autotune)))
         return
     if args.workload:
-        print(json.dumps(other_workloads(sn, log, autotune=not args.no_autotune, only=args.workload)))
+        print(json.dumps(other_workloads(sn, log, autotune=not args.no_autotune, only=args.workload,
+                                         schedule=parse_schedule(sn, args.schedule))))
         return
 
     L, rc = args.L, args.rc
@@ -222,8 +224,7 @@ def main():
     n_nb = len(sn.gen_neighbour(rc, L))
     F = flops_per_update(n_nb, 6)
     if args.schedule:
-        v = [int(x) for x in args.schedule.split(",")]
-        tun, tuned_rate = sn.TuningParams(tile=tuple(v[:3]), fine_colouring=v[3], kernel=v[4]), None
+        tun, tuned_rate = parse_schedule(sn, args.schedule), None
     elif args.no_autotune:
         tun, tuned_rate = None, None
     else:
@@ -317,6 +318,15 @@ def main():
     print(json.dumps(line))
 
 
+def parse_schedule(sn, text):
+    """TuningParams from 'tx,ty,tz,colouring,kernel[,threads]', or None."""
+    if not text:
+        return None
+    v = [int(x) for x in text.split(",")]
+    return sn.TuningParams(tile=tuple(v[:3]), fine_colouring=v[3], kernel=v[4],
+                           block_threads=v[5] if len(v) > 5 else 0)
+
+
 def _timed_sweeps(ctx, sweeps, sites):
     ctx.MC_sweeps(2)
     t0 = time.perf_counter()
@@ -325,13 +335,16 @@ def _timed_sweeps(ctx, sweeps, sites):
     return sweeps * sites / (time.perf_counter() - t0)
 
 
-def other_workloads(sn, log, autotune=True, only=None):
+def other_workloads(sn, log, autotune=True, only=None, schedule=None):
     """The remaining configurations, each timed briefly (site-updates/s of the
     sweeps; the adjoint in gradient evaluations/s).  `only` names one of them
-    (a profiler run of a single workload)."""
+    (a profiler run of a single workload); `schedule` (TuningParams) replaces
+    the autotuner's pick."""
     import torch
 
     def tuned(reps, ms, **species):
+        if schedule is not None:
+            return schedule
         return sn.autotune(reps, budget_ms=ms, use_cache=False, **species)[0] if autotune else None
 
     def square_2d():
