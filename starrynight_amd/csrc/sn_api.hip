@@ -83,6 +83,7 @@ struct sn_ctx {
     size_t stage_cap = 0;
     double *d_scratch = nullptr;   // this device's `scratch` array (not owned)
     int max_tile_sites = 0, max_cls = 0, max_chunks = 0;
+    int gather_region = 0;   // largest staged region of the gather kernel (z rows padded, gather_zstride)
     int field_region = 0;   // largest staged region of the field kernel (z rows padded to odd)
     int *d_tile_list = nullptr;          // tile indices grouped by replica (tiled energy kernel)
     int *d_tile_seg = nullptr;           // R + 1 offsets into d_tile_list (device copy)
@@ -260,7 +261,7 @@ static bool plan(const std::vector<ReplicaGeom> &geom, int rc, const sn_tuning &
 // dynamic LDS bytes of the per-phase gather kernel
 static size_t sweep_lds(const sn_ctx *c, bool mixed)
 {
-    size_t dbl = (size_t)((mixed ? 4 : 3) * c->sched.max_region + 4 * c->prop_chunk) * sizeof(double);
+    size_t dbl = (size_t)((mixed ? 4 : 3) * c->gather_region + 4 * c->prop_chunk) * sizeof(double);
     size_t ints = (size_t)(c->max_tile_sites + c->max_cls + 1 + 2 * (c->max_chunks + 1) +
                            (c->max_gsteps + 1) * (c->blk / c->G + 1)) * sizeof(int);
     return dbl + ints;
@@ -623,12 +624,13 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         cstart.insert(cstart.end(), s.class_start.begin(), s.class_start.end());
         // by fine colour: region index (tile + halo) and tile-local index of
         // each site; by tile index: packed local coordinates
-        const int RY = s.ty + 2 * c->rc, RZ = s.tz + 2 * hz;
+        const int RY = s.ty + 2 * c->rc, RZ = s.tz + 2 * hz, RZs = gather_zstride(RZ);
         for (size_t k = 0; k < s.class_sites.size(); k++) {
             int li = s.class_sites[k];
             int lz = li % s.tz, ly = (li / s.tz) % s.ty, lx = li / (s.tz * s.ty);
-            cri.push_back(((lx + c->rc) * RY + (ly + c->rc)) * RZ + (lz + hz));
+            cri.push_back(((lx + c->rc) * RY + (ly + c->rc)) * RZs + (lz + hz));
         }
+        c->gather_region = std::max(c->gather_region, (s.tx + 2 * c->rc) * RY * RZs);
         // field kernel: tile-order arrays use padded strides (z in 3D, y in
         // 2D) so that a scatter's targets spread over the LDS banks
         auto pad = [](int t) { return (t % 2 == 0) ? t + 3 : t + 2; };
@@ -754,7 +756,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     for (int m = 0; m < 2; m++) {
         const void *f = c->use_field
                             ? (const void *)pick_field(c->rc, c->two_d, m == 1)
-                            : (const void *)pick_sweep(G, NPL, m == 1, c->blk, c->sched.max_region, c->prop_chunk);
+                            : (const void *)pick_sweep(G, NPL, m == 1, c->blk, c->gather_region, c->prop_chunk);
         HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need_lds));
     }
     return SN_OK;
@@ -967,7 +969,7 @@ static SweepArgs sweep_args(sn_ctx *c)
     a.prop_stride = c->prop_chunk;
     a.pairs = c->d_pairs; a.sweep_ctr = c->d_sweep; a.counters = c->d_counters;
     a.npairs = c->npairs; a.rc = c->rc; a.two_d = c->two_d ? 1 : 0;
-    a.max_region = c->sched.max_region;
+    a.max_region = c->gather_region;
     a.max_tile_sites = c->max_tile_sites;
     a.max_cls = c->max_cls;
     return a;
@@ -989,7 +991,7 @@ static int launch_coarse(sn_ctx *c, bool mixed, int first, int n)
         hipLaunchKernelGGL(pick_field(c->rc, c->two_d, mixed), dim3(n), dim3(kBlock), field_lds(c, mixed),
                            c->stream, fa, first, n);
     } else {
-        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed, c->blk, c->sched.max_region, c->prop_chunk), dim3(n), dim3(c->blk),
+        hipLaunchKernelGGL(pick_sweep(c->G, c->NPL, mixed, c->blk, c->gather_region, c->prop_chunk), dim3(n), dim3(c->blk),
                            sweep_lds(c, mixed),
                            c->stream, sweep_args(c), first, n);
     }

@@ -28,6 +28,12 @@ struct ShapeDev {
     int gstep_off, gchunk_off, gsteps_n;   // gather kernel's step table (see sn_kernels.h)
 };
 
+// z stride of the gather kernel's staged region.  A row of a multiple of 8
+// sites (16 for 8-site tiles at DipoleCutOff 4) is padded by one: with a
+// stride of 16 doubles every neighbour at the same dz lands in the same LDS
+// bank, whatever its dx and dy.
+__host__ __device__ inline int gather_zstride(int rz) { return (rz > 1 && rz % 8 == 0) ? rz + 1 : rz; }
+
 // One +-r neighbour pair (sweep kernel) or one offset (gather kernels).
 struct PairDev {
     int dx, dy, dz, nn;

@@ -104,7 +104,8 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     const int hx = a.rc, hy = a.rc, hz = a.two_d ? 0 : a.rc;
     const int RY = sh.ty + 2 * hy, RZ = sh.tz + 2 * hz;
     const int RX = sh.tx + 2 * hx;
-    const int region = RX * RY * RZ;
+    const int region = RX * RY * RZ;                       // sites staged
+    const int RZs = gather_zstride(RZ);                    // z stride in LDS (>= RZ)
 
     const int MR = MRC > 0 ? MRC : a.max_region;          // stride of the staged arrays
     double *sx = smem;
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         int j = part + m * G;
         if (j < a.npairs) {
             PairDev p = a.pairs[j];
-            dl[m] = (p.dx * RY + p.dy) * RZ + p.dz;
+            dl[m] = (p.dx * RY + p.dy) * RZs + p.dz;
             nx[m] = p.nx; ny[m] = p.ny; nz[m] = p.nz; w[m] = p.w; w3[m] = -3.0 * p.w;
         } else {
             dl[m] = 0; nx[m] = 0.0; ny[m] = 0.0; nz[m] = 0.0; w[m] = 0.0; w3[m] = 0.0;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     const unsigned sweep = a.sweep_ctr[tile.replica];
     const long long tile_base = ((long long)tile.x0 * rep.Y + tile.y0) * rep.Z + tile.z0;
     auto draw = [&](int k, int ri) __attribute__((always_inline)) {
-        const int iz = ri % RZ, q = ri / RZ;
+        const int iz = ri % RZs, q = ri / RZs;
         const int iy = q % RY, ix = q / RY;
         const unsigned gsite = (unsigned)(tile_base + ((long long)(ix - hx) * rep.Y + (iy - hy)) * rep.Z + (iz - hz));
         double px, py, pz, u;
@@ -189,10 +190,11 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         const int sz_ = BLK % RZ, q2 = BLK / RZ, sy_ = q2 % RY, sx_ = q2 / RY;
         for (int base = tid; base < region; base += 4 * BLK) {
             double4 v[4];
-            int idx[4];
+            int idx[4], pidx[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 idx[k] = base + k * BLK;
+                pidx[k] = RZs == RZ ? idx[k] : (ix * RY + iy) * RZs + iz;
                 if (idx[k] < region) {
                     int gx = wrap_once(tile.x0 - hx + ix, rep.X);
                     int gy = wrap_once(tile.y0 - hy + iy, rep.Y);
@@ -214,8 +216,8 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (idx[k] < region) {
-                    sx[idx[k]] = v[k].x; sy[idx[k]] = v[k].y; sz[idx[k]] = v[k].z;
-                    if (MIXED) sl[idx[k]] = v[k].w;
+                    sx[pidx[k]] = v[k].x; sy[pidx[k]] = v[k].y; sz[pidx[k]] = v[k].z;
+                    if (MIXED) sl[pidx[k]] = v[k].w;
                 }
             }
         }
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     for (int i = tid; i < ntile; i += BLK) {
         int lz = i % sh.tz, q = i / sh.tz;
         int ly = q % sh.ty, lx = q / sh.ty;
-        int ri = ((lx + hx) * RY + (ly + hy)) * RZ + (lz + hz);
+        int ri = ((lx + hx) * RY + (ly + hy)) * RZs + (lz + hz);
         double4 v;
         v.x = sx[ri]; v.y = sy[ri]; v.z = sz[ri];
         v.w = MIXED ? sl[ri] : rep.L0;
