@@ -47,6 +47,11 @@ CASES = {
                                       dict(kernel=2, tile=(8, 8, 8), lanes_per_site=8)),
     "20cube_rc2_gather_region14": (dict(X=20, Y=20, Z=20, DipoleCutOff=2, T=300), True,
                                    dict(kernel=2, tile=(10, 10, 10), lanes_per_site=16)),
+    # 8-site tiles at DipoleCutOff 4: 16-site staged z rows, padded to 17 in LDS
+    "32cube_rc4_gather_padded_rows": (dict(X=32, Y=32, Z=32, DipoleCutOff=4, T=300, Efield=(0.0, 0.0, 0.03)), True,
+                                      dict(kernel=2, tile=(8, 8, 8))),
+    "32cube_rc4_gather_padded_rows_512": (dict(X=32, Y=32, Z=32, DipoleCutOff=4, T=250, K=0.1), True,
+                                          dict(kernel=2, tile=(8, 8, 8), block_threads=512)),
     # whole-lattice sweeps (kernel 3): one workgroup per replica, periodic colouring
     "25x25x1_torus": (dict(X=25, Y=25, Z=1, T=300), False, dict(kernel=3)),
     "16cube_torus_mixed_field_K": (dict(X=16, Y=16, Z=16, T=250, Efield=(0.05, 0.0, 0.1), K=0.3), True,
