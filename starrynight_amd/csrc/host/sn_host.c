@@ -469,6 +469,10 @@ int main(int argc, char **argv)
         }
         int ax = axis[0] == 'x' ? 0 : axis[0] == 'y' ? 1 : 2;
         double *acc = (double *)calloc((size_t)R, sizeof(double));
+        if (!acc) {
+            fprintf(stderr, "EfieldRamp: out of host memory\n");
+            return 1;
+        }
         printf("# step branch E_%c mean_P_%c_over_replicas stderr_over_replicas\n", "xyz"[ax], "xyz"[ax]);
         for (int b = 0; b < (loop ? 2 : 1); b++) {
             for (int k = 0; k < steps; k++) {
