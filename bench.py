@@ -157,9 +157,14 @@ def cpu_baseline(seconds_target: float = 10.0, procs: int | None = None):
     f64 = run(128, 128, 128, 3, 300, f64_moves, "f64")
     # the 2D 25x25x1 rc=3 case (the reference's CPU-runnable configuration), one core, 3000 sweeps
     sq = run(25, 25, 1, 3, 300, 3000 * 625)
+    phys = host.get("physical_cores")
     return {
         "value": sum(rates), "unit": "site-updates/s", "cores": procs, "kind": "port",
         "single_core": single["rate"],
+        # not measured: the host's other cores serve other GPUs' jobs, so this
+        # only scales the measured per-process rate to every physical core
+        "projected_all_physical_cores": (sum(rates) / procs * phys) if phys else None,
+        "physical_cores": phys,
         "single_core_fp64_oracle": f64["rate"],
         "square_25x25_rc3_single_core": sq["rate"],
         "host": host,
