@@ -45,25 +45,36 @@ __device__ __forceinline__ double dpp_f64(double v)
     return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double swz_xor16_f64(double v)
+// v + v[lane ^ 16] (ROWS = 16) or v + v[lane ^ 32] (ROWS = 32) in every lane,
+// by gfx950's v_permlane16_swap / v_permlane32_swap on both halves of the
+// double: the two results of a swap hold the even and the odd row of each
+// row pair (no LDS round trip, unlike ds_swizzle or ds_bpermute).  The sum is
+// the same in both lanes of a pair, bit for bit (addition commutes).
+template <int ROWS>
+__device__ __forceinline__ double pair_sum_f64(double v)
 {
-    // ds_swizzle bitmask mode: and 0x1F, or 0, xor 0x10 (within 32 lanes)
-    int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);
-    hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
-    return __hiloint2double(hi, lo);
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    if constexpr (ROWS == 16) {
+        const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+    } else {
+        const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+    }
 }
 
 // Sum over aligned groups of G lanes, result in every lane of the group:
 // butterfly of row_mirror (i <-> 15-i), row_half_mirror (i <-> 7-i) and
-// quad permutations xor 2, xor 1, plus a swizzle xor 16 for G >= 32 and a
-// cross-half exchange (lane xor 32) for G = 64.
+// quad permutations xor 2, xor 1, plus row-pair swaps for G >= 32 (lane xor
+// 16) and G = 64 (lane xor 32).
 template <int G>
 __device__ __forceinline__ double group_sum_dpp(double v)
 {
     static_assert(G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "G is a power of two <= 64");
-    if (G >= 64) v += __shfl_xor(v, 32, 64);
-    if (G >= 32) v += swz_xor16_f64(v);
+    if (G >= 64) v = pair_sum_f64<32>(v);
+    if (G >= 32) v = pair_sum_f64<16>(v);
     if (G >= 16) v += dpp_f64<0x140>(v);
     if (G >= 8) v += dpp_f64<0x141>(v);
     if (G >= 4) v += dpp_f64<0x4E>(v);
