@@ -121,9 +121,12 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     const int MR = MRC > 0 ? MRC : a.max_region;          // stride of the staged arrays
     double *sx = smem;
     double *sy = sx + MR;
-    double *sz = sy + MR;
-    double *sl = sz + MR;                                  // lengths (MIXED only)
-    double *prop = MIXED ? sl + MR : sl;                   // 4 x prop_stride: px, py, pz, threshold
+    // z, and with mixed lengths (z, L) pairs: the 16-byte read of a neighbour
+    // brings its length along with its z
+    constexpr int ZS = MIXED ? 2 : 1;
+    double *sz = sy + MR;                                  // z of site i at sz[ZS * i]
+    double *sl = sz + 1;                                   // L of site i at sl[2 * i] (MIXED only)
+    double *prop = sz + ZS * MR;                           // 4 x prop_stride: px, py, pz, threshold
     const int PS = MRC > 0 ? kBlock : a.prop_stride;        // (fixed-stride launches draw chunks of kBlock)
     int *s_ri = (int *)(prop + 4 * PS);                    // region index of class sites
     int *s_cls = s_ri + a.max_tile_sites;                  // class starts, then chunk colours
@@ -227,8 +230,11 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (idx[k] < region) {
-                    sx[pidx[k]] = v[k].x; sy[pidx[k]] = v[k].y; sz[pidx[k]] = v[k].z;
-                    if (MIXED) sl[pidx[k]] = v[k].w;
+                    sx[pidx[k]] = v[k].x; sy[pidx[k]] = v[k].y;
+                    if (MIXED) {
+                        double2 zl; zl.x = v[k].z; zl.y = v[k].w;
+                        ((double2 *)sz)[pidx[k]] = zl;
+                    } else sz[pidx[k]] = v[k].z;
                 }
             }
         }
@@ -253,7 +259,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         // measured 2.36e9.
         for (int i = tid; i < s_gchunk_total(sh) * SW; i += BLK) {
             const int e_ = s_gstep[i];
-            if (i % SW != 0 && e_ >= 0 && sl[e_ & 0xFFFF] == 0.0) s_gstep[i] = -1;
+            if (i % SW != 0 && e_ >= 0 && sl[2 * (e_ & 0xFFFF)] == 0.0) s_gstep[i] = -1;
         }
         __syncthreads();
         for (int st_ = tid; st_ < s_gchunk_total(sh); st_ += BLK) {
@@ -298,17 +304,23 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                 // every LDS read of the step is issued before the first use, so
                 // the step waits for LDS latency once, not once per pair
                 double ax[NPL], ay[NPL], az[NPL], bx[NPL], by[NPL], bz[NPL], la[NPL], lb[NPL];
-                const double *cx = sx + ri, *cy = sy + ri, *cz = sz + ri, *cl = sl + ri;
+                const double *cx = sx + ri, *cy = sy + ri, *cz = sz + ri;
+                const double2 *czl = (const double2 *)sz + ri;
 #pragma unroll
                 for (int m = 0; m < NPL; m++) {
-                    ax[m] = cx[dl[m]]; ay[m] = cy[dl[m]]; az[m] = cz[dl[m]];
-                    bx[m] = cx[-dl[m]]; by[m] = cy[-dl[m]]; bz[m] = cz[-dl[m]];
-                    if (MIXED) { la[m] = cl[dl[m]]; lb[m] = cl[-dl[m]]; }
+                    ax[m] = cx[dl[m]]; ay[m] = cy[dl[m]];
+                    bx[m] = cx[-dl[m]]; by[m] = cy[-dl[m]];
+                    if (MIXED) {
+                        const double2 za = czl[dl[m]], zb = czl[-dl[m]];
+                        az[m] = za.x; la[m] = za.y; bz[m] = zb.x; lb[m] = zb.y;
+                    } else {
+                        az[m] = cz[dl[m]]; bz[m] = cz[-dl[m]];
+                    }
                 }
                 const double px = prop[pq], py = prop[PS + pq], pz = prop[2 * PS + pq];
                 const double thr = prop[3 * PS + pq];
-                const double ox = sx[ri], oy = sy[ri], oz = sz[ri];
-                const double Li = MIXED ? sl[ri] : rep.L0;
+                const double ox = sx[ri], oy = sy[ri], oz = sz[ZS * ri];
+                const double Li = MIXED ? sl[2 * ri] : rep.L0;
                 asm volatile("" ::: "memory");
 #pragma unroll
                 for (int m = 0; m < NPL; m++) {
@@ -359,7 +371,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                 const bool acc = dE < thr;
                 if (part == 0 && Li != 0.0) {
                     if (acc) {
-                        sx[ri] = px; sy[ri] = py; sz[ri] = pz;
+                        sx[ri] = px; sy[ri] = py; sz[ZS * ri] = pz;
                         nacc++;
                     } else nrej++;
                 }
@@ -377,8 +389,8 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         int ly = q % sh.ty, lx = q / sh.ty;
         int ri = ((lx + hx) * RY + (ly + hy)) * RZs + (lz + hz);
         double4 v;
-        v.x = sx[ri]; v.y = sy[ri]; v.z = sz[ri];
-        v.w = MIXED ? sl[ri] : rep.L0;
+        v.x = sx[ri]; v.y = sy[ri]; v.z = sz[ZS * ri];
+        v.w = MIXED ? sl[2 * ri] : rep.L0;
         out[tile_base + ((long long)lx * rep.Y + ly) * rep.Z + lz] = v;
     }
     // ---- counters: this tile's own slot (one workgroup per tile per launch) ----
