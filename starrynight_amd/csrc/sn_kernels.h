@@ -119,12 +119,16 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
     const int RZs = gather_zstride(RZ);                    // z stride in LDS (>= RZ)
 
     const int MR = MRC > 0 ? MRC : a.max_region;          // stride of the staged arrays
-    double *sx = smem;
-    double *sy = sx + MR;
-    // z, and with mixed lengths (z, L) pairs: the 16-byte read of a neighbour
-    // brings its length along with its z
-    constexpr int ZS = MIXED ? 2 : 1;
-    double *sz = sy + MR;                                  // z of site i at sz[ZS * i]
+    // Uniform lengths: x, y and z arrays (the fixed-stride instances reach y
+    // and z by immediate offsets from the x address).  Mixed lengths: (x, y)
+    // pairs and (z, L) pairs, so two 16-byte reads bring a neighbour's
+    // dipole and length (measured at DipoleCutOff 4, half vacant: three
+    // 8-byte reads and a length read 2.49e9, (z, L) pairs 2.68e9, both pairs
+    // 2.81e9 site-updates/s; the uniform instances lost 2% with (x, y) pairs).
+    constexpr int ZS = MIXED ? 2 : 1;                      // doubles per site in the x, y and z arrays
+    double *sx = smem;                                     // x of site i at sx[ZS * i]
+    double *sy = MIXED ? sx + 1 : sx + MR;                 // y of site i at sy[ZS * i]
+    double *sz = sx + 2 * MR;                              // z of site i at sz[ZS * i]
     double *sl = sz + 1;                                   // L of site i at sl[2 * i] (MIXED only)
     double *prop = sz + ZS * MR;                           // 4 x prop_stride: px, py, pz, threshold
     const int PS = MRC > 0 ? kBlock : a.prop_stride;        // (fixed-stride launches draw chunks of kBlock)
@@ -230,11 +234,14 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (idx[k] < region) {
-                    sx[pidx[k]] = v[k].x; sy[pidx[k]] = v[k].y;
                     if (MIXED) {
-                        double2 zl; zl.x = v[k].z; zl.y = v[k].w;
+                        double2 xy, zl;
+                        xy.x = v[k].x; xy.y = v[k].y; zl.x = v[k].z; zl.y = v[k].w;
+                        ((double2 *)sx)[pidx[k]] = xy;
                         ((double2 *)sz)[pidx[k]] = zl;
-                    } else sz[pidx[k]] = v[k].z;
+                    } else {
+                        sx[pidx[k]] = v[k].x; sy[pidx[k]] = v[k].y; sz[pidx[k]] = v[k].z;
+                    }
                 }
             }
         }
@@ -305,21 +312,22 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                 // the step waits for LDS latency once, not once per pair
                 double ax[NPL], ay[NPL], az[NPL], bx[NPL], by[NPL], bz[NPL], la[NPL], lb[NPL];
                 const double *cx = sx + ri, *cy = sy + ri, *cz = sz + ri;
-                const double2 *czl = (const double2 *)sz + ri;
+                const double2 *cxy = (const double2 *)sx + ri, *czl = (const double2 *)sz + ri;
 #pragma unroll
                 for (int m = 0; m < NPL; m++) {
-                    ax[m] = cx[dl[m]]; ay[m] = cy[dl[m]];
-                    bx[m] = cx[-dl[m]]; by[m] = cy[-dl[m]];
                     if (MIXED) {
+                        const double2 pa = cxy[dl[m]], pb = cxy[-dl[m]];
                         const double2 za = czl[dl[m]], zb = czl[-dl[m]];
-                        az[m] = za.x; la[m] = za.y; bz[m] = zb.x; lb[m] = zb.y;
+                        ax[m] = pa.x; ay[m] = pa.y; az[m] = za.x; la[m] = za.y;
+                        bx[m] = pb.x; by[m] = pb.y; bz[m] = zb.x; lb[m] = zb.y;
                     } else {
-                        az[m] = cz[dl[m]]; bz[m] = cz[-dl[m]];
+                        ax[m] = cx[dl[m]]; ay[m] = cy[dl[m]]; az[m] = cz[dl[m]];
+                        bx[m] = cx[-dl[m]]; by[m] = cy[-dl[m]]; bz[m] = cz[-dl[m]];
                     }
                 }
                 const double px = prop[pq], py = prop[PS + pq], pz = prop[2 * PS + pq];
                 const double thr = prop[3 * PS + pq];
-                const double ox = sx[ri], oy = sy[ri], oz = sz[ZS * ri];
+                const double ox = sx[ZS * ri], oy = sy[ZS * ri], oz = sz[ZS * ri];
                 const double Li = MIXED ? sl[2 * ri] : rep.L0;
                 asm volatile("" ::: "memory");
 #pragma unroll
@@ -371,7 +379,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
                 const bool acc = dE < thr;
                 if (part == 0 && Li != 0.0) {
                     if (acc) {
-                        sx[ri] = px; sy[ri] = py; sz[ZS * ri] = pz;
+                        sx[ZS * ri] = px; sy[ZS * ri] = py; sz[ZS * ri] = pz;
                         nacc++;
                     } else nrej++;
                 }
@@ -389,7 +397,7 @@ __global__ __launch_bounds__(BLK) void tile_sweep_kernel(SweepArgs a, int first_
         int ly = q % sh.ty, lx = q / sh.ty;
         int ri = ((lx + hx) * RY + (ly + hy)) * RZs + (lz + hz);
         double4 v;
-        v.x = sx[ri]; v.y = sy[ri]; v.z = sz[ZS * ri];
+        v.x = sx[ZS * ri]; v.y = sy[ZS * ri]; v.z = sz[ZS * ri];
         v.w = MIXED ? sl[2 * ri] : rep.L0;
         out[tile_base + ((long long)lx * rep.Y + ly) * rep.Z + lz] = v;
     }
