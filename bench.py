@@ -353,11 +353,21 @@ def other_workloads(sn, log, autotune=True, only=None, schedule=None):
         return sn.autotune(reps, budget_ms=ms, use_cache=False, **species)[0] if autotune else None
 
     def square_2d():
-        # 2D 25x25x1, rc = 3, T = 300 (the reference's CPU-runnable case): 1000 sweeps
+        # 2D 25x25x1, rc = 3, T = 300 (the reference's CPU-runnable case): 1000 sweeps.
+        # One 625-site lattice fills one workgroup on one CU; the batched form
+        # (512 replicas, 50-500 K, the library's own kernel choice) fills the device.
         rp = sn.ReplicaParams(25, 25, 1, DipoleCutOff=3, T=300, seed=0xDEADBEEF + 300)
         with sn.StarryNight(rp, tuned(rp, 1500)) as ctx:
             ctx.init_lattice_random(0)
-            return {"site_updates_per_s": _timed_sweeps(ctx, 1000, rp.N), "sweeps": 1000}
+            single = _timed_sweeps(ctx, 1000, rp.N)
+        reps = [sn.ReplicaParams(25, 25, 1, DipoleCutOff=3, T=50.0 + 450.0 * k / 511, seed=2500 + k)
+                for k in range(512)]
+        with sn.StarryNight(reps) as ctx:
+            for r in range(len(reps)):
+                ctx.init_lattice_random(r)
+            agg = _timed_sweeps(ctx, 200, sum(r.N for r in reps))
+        return {"site_updates_per_s": single, "sweeps": 1000,
+                "batched_512_replicas_site_updates_per_s": agg, "batched_sweeps": 200}
 
     def temperature_sweep():
         # 64^3 rc = 3: one replica, and the 0-500 K sweep in 25 K steps batched as 21 replicas
