@@ -243,6 +243,8 @@ constexpr int kMaxRegion = 4096;
 // Replicas of at most this many sites sweep with the whole-lattice kernel
 // when the tuning leaves the kernel to the library.
 constexpr long long kAutoTorusSites = 2048;
+// pair count above which the gather kernel defaults to 512 threads
+constexpr int kWidePairs = 128;
 // Largest tile: the field kernel gives each lane at most 2 items of 2 sites
 // (3D) or 1 site (2D).
 constexpr int kMaxTile3D = kFieldMaxTile3D, kMaxTile2D = kFieldMaxTile2D;
@@ -563,27 +565,44 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     // lanes per site and pairs per lane
     int max_class = 1;
     for (const TileShape &s : c->sched.shapes) max_class = std::max(max_class, s.max_class);
-    const int blk = c->tuning.block_threads > 0 ? c->tuning.block_threads : kBlock;
+    int blk = c->tuning.block_threads > 0 ? c->tuning.block_threads : kBlock;
     if (blk != 256 && blk != 512) return fail(c, SN_ERR_INVALID, "block_threads must be 256 or 512");
     if (c->tuning.kernel < 0 || c->tuning.kernel > 3) return fail(c, SN_ERR_INVALID, "kernel must be 0..3");
     // kernel 0 picks by size (rates measured on MI355X): the whole-lattice
     // kernel for replicas of up to kAutoTorusSites sites (1.6-3x the gather
     // kernel at 12^3 and 25x25), else the per-phase gather kernel (1.2-2x the
     // field-update kernel from 16^3 to 128^3 at DipoleCutOff 2-4, batched or
-    // not), else the field-update kernel where no gather instance fits
+    // not; 2.1-2.3x at DipoleCutOff 1 and 1.35x at 5, 64^3 and 128^3), else
+    // the field-update kernel where no gather instance fits
     int kern = c->tuning.kernel;
     if (kern == 0 && blk != kBlock) kern = 2;
     if (blk != kBlock && kern != 2)
         return fail(c, SN_ERR_INVALID, "block_threads applies to the gather kernel (kernel = 2) only");
-    c->blk = blk;
-    int G = c->tuning.lanes_per_site;
-    if (G <= 0) {
-        G = blk == 512 ? 64 : 32;
-        while (G > 4 && (blk / G) < max_class) G >>= 1;
+    auto lane_shape = [&](int threads, int &G, int &NPL) {
+        G = c->tuning.lanes_per_site;
+        if (G <= 0) {
+            G = threads == 512 ? 64 : 32;
+            while (G > 4 && (threads / G) < max_class) G >>= 1;
+        }
+        const int need = (c->npairs + G - 1) / G;
+        NPL = 2;
+        while (NPL < need) NPL <<= 1;
+    };
+    int G, NPL;
+    lane_shape(blk, G, NPL);
+    // The gather kernel runs 512 threads, unless the tuning fixes the block
+    // or the lanes, beyond 128 pairs (DipoleCutOff 5: 64 lanes a site, 1.35x
+    // the field-update kernel at 64^3 and 128^3, where 256 threads of 32
+    // lanes x 16 pairs, one wave per SIMD, run at half that kernel's rate)
+    // and where 256 threads at 4 lanes a site hold fewer sites than a colour
+    // class (DipoleCutOff 1: 1.08-1.10x the 256-thread rate).
+    const bool wide = c->npairs > kWidePairs || max_class > kBlock / 4;
+    if (c->tuning.block_threads <= 0 && c->tuning.lanes_per_site <= 0 && (kern == 0 || kern == 2) && wide) {
+        int G2, NPL2;
+        lane_shape(512, G2, NPL2);
+        if (pick_sweep(G2, NPL2, false, 512)) { blk = 512; G = G2; NPL = NPL2; }
     }
-    int need = (c->npairs + G - 1) / G;
-    int NPL = 2;
-    while (NPL < need) NPL <<= 1;
+    c->blk = blk;
     c->G = G; c->NPL = NPL;
     const bool field_ok = pick_field(c->rc, c->two_d, false) != nullptr;
     const bool gather_ok = pick_sweep(G, NPL, false, blk) != nullptr;
@@ -595,7 +614,11 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
         kern = (gather_ok || !field_ok) ? 2 : 1;
     }
     if (c->can_sweep && (kern == 2 || !field_ok) && !gather_ok)
-        return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff");
+        return fail(c, SN_ERR_UNSUPPORTED, "no tile-sweep kernel for this lanes-per-site / cutoff (" +
+                                               std::to_string(G) + " lanes x " + std::to_string(NPL) +
+                                               " pairs per lane, " + std::to_string(blk) + " threads; " +
+                                               std::to_string(c->npairs) + " pairs, colour classes of up to " +
+                                               std::to_string(max_class) + " sites)");
 
     HIPCHK(c, hipGetDevice(&c->device));
     if (int se = resolve_scratch(c)) return se;
@@ -1631,7 +1654,7 @@ int sn_sweep_config(sn_ctx *c, int32_t *tiles, int32_t *launches, int32_t *lanes
     if (c->torus) { nl = 1; maxcol = c->torus_ncol; }
     if (tiles) *tiles = c->torus ? 0 : (int32_t)c->sched.tiles.size();
     if (launches) *launches = nl;
-    if (lanes) *lanes = c->torus ? c->torus_G : c->G;
+    if (lanes) *lanes = c->torus ? c->torus_G : c->use_field ? kFieldG : c->G;
     if (fine) *fine = maxcol;
     return SN_OK;
 }

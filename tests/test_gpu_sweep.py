@@ -47,6 +47,21 @@ CASES = {
                                       dict(kernel=2, tile=(8, 8, 8), lanes_per_site=8)),
     "20cube_rc2_gather_region14": (dict(X=20, Y=20, Z=20, DipoleCutOff=2, T=300), True,
                                    dict(kernel=2, tile=(10, 10, 10), lanes_per_site=16)),
+    # the gather kernel (the library's choice above 2048 sites) on the field kernel's cases
+    "16cube_gather_constrain_x": (dict(X=16, Y=16, Z=16, T=300, ConstrainToX=True), True, dict(kernel=2)),
+    "16cube_gather_T0": (dict(X=16, Y=16, Z=16, T=0), True, dict(kernel=2)),
+    "odd_tiles_17x19x21_gather": (dict(X=17, Y=19, Z=21, T=350, Efield=(0.03, 0.0, 0.0)), True, dict(kernel=2)),
+    "12cube_rc1_gather": (dict(X=12, Y=12, Z=12, DipoleCutOff=1, T=300), True, dict(kernel=2)),
+    "20cube_rc2_gather_512": (dict(X=20, Y=20, Z=20, DipoleCutOff=2, T=300, K=0.1), True,
+                              dict(kernel=2, block_threads=512)),
+    "24cube_rc5_gather": (dict(X=24, Y=24, Z=24, DipoleCutOff=5, T=300), True, dict(kernel=2)),
+    "24cube_rc5_gather_512": (dict(X=24, Y=24, Z=24, DipoleCutOff=5, T=300), False,
+                              dict(kernel=2, block_threads=512)),
+    "12cube_rc1_gather_512": (dict(X=12, Y=12, Z=12, DipoleCutOff=1, T=300, Efield=(0, 0, 0.05)), False,
+                              dict(kernel=2, block_threads=512)),
+    "2d_mixed_30x26_gather": (dict(X=30, Y=26, Z=1, T=200, K=0.2, Efield=(0.02, -0.01, 0.0)), True,
+                              dict(kernel=2)),
+    "12x16x20_gather_J2": (dict(X=12, Y=16, Z=20, T=400, CageStrain=2.0), True, dict(kernel=2)),
     # 8-site tiles at DipoleCutOff 4: 16-site staged z rows, padded to 17 in LDS
     "32cube_rc4_gather_padded_rows": (dict(X=32, Y=32, Z=32, DipoleCutOff=4, T=300, Efield=(0.0, 0.0, 0.03)), True,
                                       dict(kernel=2, tile=(8, 8, 8))),
@@ -231,7 +246,8 @@ def test_untileable_lattice_refuses_sweeps_but_gathers_work(sn, oracle):
 
 def test_default_kernel_choice(sn):
     """Kernel 0 (and no tuning at all): whole-lattice sweeps for replicas of
-    up to 2048 sites, tiles with the gather kernel above that.  9^3 has no
+    up to 2048 sites, tiles with the gather kernel above that, at 512 threads
+    for DipoleCutOff 5.  9^3 has no
     tiling at rc = 3 (tile kernels refuse it) but a periodic colouring."""
     cases = [([sn.ReplicaParams(12, 12, 12, T=300)], True),
              ([sn.ReplicaParams(9, 9, 9, T=300)], True),
@@ -245,6 +261,15 @@ def test_default_kernel_choice(sn):
                 assert (cfg["tiles"] == 0) == whole, (reps[0].X, cfg)
                 if not whole:
                     assert cfg["lanes_per_site"] == 16
+    # DipoleCutOff 1: 4 lanes a site (3 pairs); 5: 512 threads, 64 lanes a
+    # site (257 pairs); kernel 1 reports the field kernel's 16 lanes
+    for rc, tun, lanes in ((1, None, 4), (1, sn.TuningParams(kernel=2, block_threads=256), 4),
+                           (5, None, 64), (5, sn.TuningParams(kernel=2), 64),
+                           (5, sn.TuningParams(kernel=2, block_threads=256), 32),
+                           (5, sn.TuningParams(kernel=1), 16)):
+        with sn.StarryNight(sn.ReplicaParams(32, 32, 32, DipoleCutOff=rc, T=300), tun) as ctx:
+            cfg = ctx.sweep_config()
+            assert cfg["tiles"] > 0 and cfg["lanes_per_site"] == lanes, (rc, tun, cfg)
 
 
 def test_field_and_gather_kernels_agree(sn):
