@@ -83,6 +83,10 @@ CASES = {
                                      dict(kernel=3)),
     "11x13x1_torus_greedy": (dict(X=11, Y=13, Z=1, T=300), False, dict(kernel=3)),
     "25x25x1_torus_lanes8": (dict(X=25, Y=25, Z=1, T=300), True, dict(kernel=3, lanes_per_site=8)),
+    "12cube_torus_T0_field": (dict(X=12, Y=12, Z=12, T=0, Efield=(0.0, 0.0, -0.05)), True, dict(kernel=3)),
+    "10cube_rc1_torus": (dict(X=10, Y=10, Z=10, DipoleCutOff=1, T=300, K=0.1), True, dict(kernel=3)),
+    "24x20x1_rc5_torus": (dict(X=24, Y=20, Z=1, DipoleCutOff=5, T=300), True, dict(kernel=3)),
+    "12cube_rc1_auto": (dict(X=12, Y=12, Z=12, DipoleCutOff=1, T=300), True, {}),
 }
 
 
