@@ -243,7 +243,7 @@ constexpr int kMaxRegion = 4096;
 // Replicas of at most this many sites sweep with the whole-lattice kernel
 // when the tuning leaves the kernel to the library.
 constexpr long long kAutoTorusSites = 2048;
-// pair count above which the gather kernel defaults to 512 threads
+// pair count above which the gather kernel gives a site 64 lanes
 constexpr int kWidePairs = 128;
 // Largest tile: the field kernel gives each lane at most 2 items of 2 sites
 // (3D) or 1 site (2D).
@@ -572,7 +572,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     // kernel for replicas of up to kAutoTorusSites sites (1.6-3x the gather
     // kernel at 12^3 and 25x25), else the per-phase gather kernel (1.2-2x the
     // field-update kernel from 16^3 to 128^3 at DipoleCutOff 2-4, batched or
-    // not; 2.1-2.3x at DipoleCutOff 1 and 1.35x at 5, 64^3 and 128^3), else
+    // not; 2.3-2.6x at DipoleCutOff 1 and 1.55x at 5, 64^3 and 128^3), else
     // the field-update kernel where no gather instance fits
     int kern = c->tuning.kernel;
     if (kern == 0 && blk != kBlock) kern = 2;
@@ -581,22 +581,27 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     auto lane_shape = [&](int threads, int &G, int &NPL) {
         G = c->tuning.lanes_per_site;
         if (G <= 0) {
-            G = threads == 512 ? 64 : 32;
+            G = threads == 512 || c->npairs > kWidePairs ? 64 : 32;
             while (G > 4 && (threads / G) < max_class) G >>= 1;
         }
+        // pairs per lane: exactly enough where an instance has that count,
+        // else the next power of two (padded slots carry zero weight)
         const int need = (c->npairs + G - 1) / G;
-        NPL = 2;
-        while (NPL < need) NPL <<= 1;
+        NPL = need;
+        if (!pick_sweep(G, NPL, false, threads)) {
+            NPL = 2;
+            while (NPL < need) NPL <<= 1;
+        }
     };
     int G, NPL;
     lane_shape(blk, G, NPL);
     // The gather kernel runs 512 threads, unless the tuning fixes the block
-    // or the lanes, beyond 128 pairs (DipoleCutOff 5: 64 lanes a site, 1.35x
-    // the field-update kernel at 64^3 and 128^3, where 256 threads of 32
-    // lanes x 16 pairs, one wave per SIMD, run at half that kernel's rate)
-    // and where 256 threads at 4 lanes a site hold fewer sites than a colour
-    // class (DipoleCutOff 1: 1.08-1.10x the 256-thread rate).
-    const bool wide = c->npairs > kWidePairs || max_class > kBlock / 4;
+    // or the lanes, where 256 threads at 4 lanes a site hold fewer sites than
+    // a colour class (DipoleCutOff 1: 4 lanes x 1 pair, 1.06-1.09x the
+    // 256-thread rate at 64^3 and 128^3).  Beyond 128 pairs (DipoleCutOff 5)
+    // a site takes 64 lanes of 5 pairs at 256 threads: 1.16x 512 threads of
+    // 64 lanes x 8 pairs, 1.56x the field-update kernel at 128^3.
+    const bool wide = max_class > kBlock / 4;
     if (c->tuning.block_threads <= 0 && c->tuning.lanes_per_site <= 0 && (kern == 0 || kern == 2) && wide) {
         int G2, NPL2;
         lane_shape(512, G2, NPL2);

@@ -59,6 +59,11 @@ CASES = {
                               dict(kernel=2, block_threads=512)),
     "12cube_rc1_gather_512": (dict(X=12, Y=12, Z=12, DipoleCutOff=1, T=300, Efield=(0, 0, 0.05)), False,
                               dict(kernel=2, block_threads=512)),
+    # exact pair counts per lane: 4 lanes x 1 pair (3 pairs), 64 lanes x 5 pairs (257 pairs)
+    "12cube_rc1_gather_256": (dict(X=12, Y=12, Z=12, DipoleCutOff=1, T=300, K=0.1), True,
+                              dict(kernel=2, block_threads=256)),
+    "24cube_rc5_gather_256_lanes64": (dict(X=24, Y=24, Z=24, DipoleCutOff=5, T=300, Efield=(0.0, 0.02, 0.0)), True,
+                                      dict(kernel=2, block_threads=256, lanes_per_site=64)),
     "2d_mixed_30x26_gather": (dict(X=30, Y=26, Z=1, T=200, K=0.2, Efield=(0.02, -0.01, 0.0)), True,
                               dict(kernel=2)),
     "12x16x20_gather_J2": (dict(X=12, Y=16, Z=20, T=400, CageStrain=2.0), True, dict(kernel=2)),
@@ -250,8 +255,8 @@ def test_untileable_lattice_refuses_sweeps_but_gathers_work(sn, oracle):
 
 def test_default_kernel_choice(sn):
     """Kernel 0 (and no tuning at all): whole-lattice sweeps for replicas of
-    up to 2048 sites, tiles with the gather kernel above that, at 512 threads
-    for DipoleCutOff 5.  9^3 has no
+    up to 2048 sites, tiles with the gather kernel above that (at 512 threads
+    for DipoleCutOff 1).  9^3 has no
     tiling at rc = 3 (tile kernels refuse it) but a periodic colouring."""
     cases = [([sn.ReplicaParams(12, 12, 12, T=300)], True),
              ([sn.ReplicaParams(9, 9, 9, T=300)], True),
@@ -265,11 +270,11 @@ def test_default_kernel_choice(sn):
                 assert (cfg["tiles"] == 0) == whole, (reps[0].X, cfg)
                 if not whole:
                     assert cfg["lanes_per_site"] == 16
-    # DipoleCutOff 1: 4 lanes a site (3 pairs); 5: 512 threads, 64 lanes a
-    # site (257 pairs); kernel 1 reports the field kernel's 16 lanes
+    # DipoleCutOff 1: 4 lanes a site (3 pairs); 5: 64 lanes a site (257
+    # pairs, 5 a lane); kernel 1 reports the field kernel's 16 lanes
     for rc, tun, lanes in ((1, None, 4), (1, sn.TuningParams(kernel=2, block_threads=256), 4),
                            (5, None, 64), (5, sn.TuningParams(kernel=2), 64),
-                           (5, sn.TuningParams(kernel=2, block_threads=256), 32),
+                           (5, sn.TuningParams(kernel=2, block_threads=256), 64),
                            (5, sn.TuningParams(kernel=1), 16)):
         with sn.StarryNight(sn.ReplicaParams(32, 32, 32, DipoleCutOff=rc, T=300), tun) as ctx:
             cfg = ctx.sweep_config()
