@@ -243,8 +243,6 @@ constexpr int kMaxRegion = 4096;
 // Replicas of at most this many sites sweep with the whole-lattice kernel
 // when the tuning leaves the kernel to the library.
 constexpr long long kAutoTorusSites = 2048;
-// pair count above which the gather kernel gives a site 64 lanes
-constexpr int kWidePairs = 128;
 // Largest tile: the field kernel gives each lane at most 2 items of 2 sites
 // (3D) or 1 site (2D).
 constexpr int kMaxTile3D = kFieldMaxTile3D, kMaxTile2D = kFieldMaxTile2D;
@@ -578,29 +576,51 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if (kern == 0 && blk != kBlock) kern = 2;
     if (blk != kBlock && kern != 2)
         return fail(c, SN_ERR_INVALID, "block_threads applies to the gather kernel (kernel = 2) only");
+    // pairs per lane for G lanes a site: exactly enough where an instance
+    // has that count, else the next power of two (padded slots carry zero
+    // weight)
+    auto pairs_per_lane = [&](int G, int threads) {
+        const int need = (c->npairs + G - 1) / G;
+        if (pick_sweep(G, need, false, threads)) return need;
+        int n = 2;
+        while (n < need) n <<= 1;
+        return n;
+    };
+    // Lanes per site: the G whose groups hold a colour class in one step
+    // (down to 4 lanes) with the fewest pair slots G x NPL, the larger G on
+    // a tie.  Measured at 1024^2 (2D), site-updates/s: DipoleCutOff 3 (14
+    // pairs) 16 x 1 5.8e9 against 32 x 2 4.7e9; DipoleCutOff 4 (24 pairs)
+    // 16 x 2 4.8e9 against 32 x 2 4.3e9; DipoleCutOff 2 (6 pairs) 4 x 2
+    // 6.8e9 against 16 x 1 6.7e9.  3D lattices keep 16 x 4 (61 pairs) and
+    // 16 x 8 (128), and DipoleCutOff 5 (257 pairs) takes 64 x 5.
     auto lane_shape = [&](int threads, int &G, int &NPL) {
         G = c->tuning.lanes_per_site;
-        if (G <= 0) {
-            G = threads == 512 || c->npairs > kWidePairs ? 64 : 32;
-            while (G > 4 && (threads / G) < max_class) G >>= 1;
+        if (G > 0) {
+            NPL = pairs_per_lane(G, threads);
+            return;
         }
-        // pairs per lane: exactly enough where an instance has that count,
-        // else the next power of two (padded slots carry zero weight)
-        const int need = (c->npairs + G - 1) / G;
-        NPL = need;
-        if (!pick_sweep(G, NPL, false, threads)) {
-            NPL = 2;
-            while (NPL < need) NPL <<= 1;
+        int best = 0, best_slots = 0;
+        for (int g = 64; g >= 4; g >>= 1) {
+            if (g > 4 && threads / g < max_class) continue;
+            const int n = pairs_per_lane(g, threads);
+            if (!pick_sweep(g, n, false, threads)) continue;
+            if (best == 0 || g * n < best_slots) { best = g; best_slots = g * n; }
         }
+        if (best == 0) {   // no instance: the shape the error message names
+            best = 64;
+            while (best > 4 && threads / best < max_class) best >>= 1;
+        }
+        G = best;
+        NPL = pairs_per_lane(G, threads);
     };
     int G, NPL;
     lane_shape(blk, G, NPL);
     // The gather kernel runs 512 threads, unless the tuning fixes the block
     // or the lanes, where 256 threads at 4 lanes a site hold fewer sites than
     // a colour class (DipoleCutOff 1: 4 lanes x 1 pair, 1.06-1.09x the
-    // 256-thread rate at 64^3 and 128^3).  Beyond 128 pairs (DipoleCutOff 5)
-    // a site takes 64 lanes of 5 pairs at 256 threads: 1.16x 512 threads of
-    // 64 lanes x 8 pairs, 1.56x the field-update kernel at 128^3.
+    // 256-thread rate at 64^3 and 128^3).  DipoleCutOff 5 stays at 256
+    // threads, 64 lanes x 5 pairs: 1.16x 512 threads of 64 lanes x 8 pairs,
+    // 1.55x the field-update kernel at 128^3.
     const bool wide = max_class > kBlock / 4;
     if (c->tuning.block_threads <= 0 && c->tuning.lanes_per_site <= 0 && (kern == 0 || kern == 2) && wide) {
         int G2, NPL2;

@@ -25,7 +25,7 @@ SweepFn pick_sweep(int G, int NPL, bool mixed, int blk, int max_region, int prop
     // DipoleCutOff 1 (3 pairs, colour classes of 100+ sites) and 5 (257 pairs),
     // DipoleCutOff 2 at 512 threads (16 pairs, classes of up to 64 sites)
     SN_CASE(4, 2, 256) SN_CASE(4, 2, 512) SN_CASE(32, 16, 256) SN_CASE(64, 8, 512) SN_CASE(8, 2, 512)
-    SN_CASE(4, 1, 256) SN_CASE(4, 1, 512) SN_CASE(64, 5, 256)
+    SN_CASE(4, 1, 256) SN_CASE(4, 1, 512) SN_CASE(64, 5, 256) SN_CASE(16, 1, 256)
 #undef SN_CASE
     return nullptr;
 }
