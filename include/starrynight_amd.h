@@ -57,7 +57,7 @@ typedef struct sn_tuning {
     int32_t lanes_per_site;   /* lanes that share one site's neighbour sum: 2..64 */
     int32_t fine_colouring;   /* 0 auto, 1 cubic (rc+1)^3, 2 min-index sublattice, 3 BCC (rc=3) */
     int32_t use_graph;        /* 0 or 1: replay sweeps from a captured hipGraph; -1: direct launches */
-    int32_t kernel;           /* 0 auto (kernel 3 for replicas of up to 2048 sites when it can
+    int32_t kernel;           /* 0 auto (kernel 3 for replicas of up to 4096 sites when it can
                                  colour them, else kernel 2; NULL tuning is the same),
                                  1 field-update tile kernel, 2 per-phase gather tile kernel
                                  (both visit sites in the same order with the same draws),

@@ -241,8 +241,11 @@ static void launch_reduce(sn_ctx *c, int nblk, int width)
 // plus the proposal slots and shape tables stay under the 160 KB of a CU.
 constexpr int kMaxRegion = 4096;
 // Replicas of at most this many sites sweep with the whole-lattice kernel
-// when the tuning leaves the kernel to the library.
-constexpr long long kAutoTorusSites = 2048;
+// when the tuning leaves the kernel to the library (and it fits in LDS).
+// Measured against gather tiles, site-updates/s at DipoleCutOff 3: 14^3
+// +40-44%, 16 x 16 x 12 +80%, 16^3 +0-6% (one or 64 replicas); 16^3 at
+// DipoleCutOff 4 +21-25% (5-10% half vacant), at 3 half vacant -7-10%.
+constexpr long long kAutoTorusSites = 4096;
 // Largest tile: the field kernel gives each lane at most 2 items of 2 sites
 // (3D) or 1 site (2D).
 constexpr int kMaxTile3D = kFieldMaxTile3D, kMaxTile2D = kFieldMaxTile2D;
@@ -567,7 +570,7 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     if (blk != 256 && blk != 512) return fail(c, SN_ERR_INVALID, "block_threads must be 256 or 512");
     if (c->tuning.kernel < 0 || c->tuning.kernel > 3) return fail(c, SN_ERR_INVALID, "kernel must be 0..3");
     // kernel 0 picks by size (rates measured on MI355X): the whole-lattice
-    // kernel for replicas of up to kAutoTorusSites sites (1.6-3x the gather
+    // kernel for replicas of up to kAutoTorusSites sites (2-3x the gather
     // kernel at 12^3 and 25x25), else the per-phase gather kernel (1.2-2x the
     // field-update kernel from 16^3 to 128^3 at DipoleCutOff 2-4, batched or
     // not; 2.3-2.6x at DipoleCutOff 1 and 1.55x at 5, 64^3 and 128^3), else

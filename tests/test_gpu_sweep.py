@@ -47,7 +47,7 @@ CASES = {
                                       dict(kernel=2, tile=(8, 8, 8), lanes_per_site=8)),
     "20cube_rc2_gather_region14": (dict(X=20, Y=20, Z=20, DipoleCutOff=2, T=300), True,
                                    dict(kernel=2, tile=(10, 10, 10), lanes_per_site=16)),
-    # the gather kernel (the library's choice above 2048 sites) on the field kernel's cases
+    # the gather kernel (the library's choice above 4096 sites) on the field kernel's cases
     "16cube_gather_constrain_x": (dict(X=16, Y=16, Z=16, T=300, ConstrainToX=True), True, dict(kernel=2)),
     "16cube_gather_T0": (dict(X=16, Y=16, Z=16, T=0), True, dict(kernel=2)),
     "odd_tiles_17x19x21_gather": (dict(X=17, Y=19, Z=21, T=350, Efield=(0.03, 0.0, 0.0)), True, dict(kernel=2)),
@@ -178,18 +178,23 @@ def test_batched_uneven_replicas_parity(sn, oracle):
 
 
 def test_batched_equals_single(sn):
-    """A replica's trajectory does not depend on what else is batched with it."""
+    """With tiles, a replica's trajectory does not depend on what else is
+    batched with it.  (Whole-lattice sweeps colour a batch with one
+    sublattice that holds every box, so their order may change with the
+    batch; the library's own choice for a lone 16^3 replica is such sweeps,
+    hence the pinned kernel.)"""
     a = sn.ReplicaParams(16, 16, 16, T=300, seed=99)
     b = sn.ReplicaParams(20, 20, 20, T=150, seed=98)
     lat = rand_lattice(a.N, np.random.default_rng(3))
-    with sn.StarryNight([a]) as c1:
+    tiles = sn.TuningParams(kernel=2)
+    with sn.StarryNight([a], tiles) as c1:
         c1.set_lattice(lat); c1.MC_sweeps(5); one = c1.get_lattice()
-    with sn.StarryNight([b, a]) as c2:
+    with sn.StarryNight([b, a], tiles) as c2:
         c2.set_lattice(rand_lattice(b.N, np.random.default_rng(4)), 0)
         c2.set_lattice(lat, 1); c2.MC_sweeps(5); two = c2.get_lattice(1)
     # same schedule for `a`, replica index differs -> different RNG stream unless
     # the replica slot matches; compare with the replica in slot 0 instead
-    with sn.StarryNight([a, b]) as c3:
+    with sn.StarryNight([a, b], tiles) as c3:
         c3.set_lattice(lat, 0); c3.set_lattice(rand_lattice(b.N, np.random.default_rng(4)), 1)
         c3.MC_sweeps(5); three = c3.get_lattice(0)
     assert np.array_equal(one, three)
@@ -255,20 +260,23 @@ def test_untileable_lattice_refuses_sweeps_but_gathers_work(sn, oracle):
 
 def test_default_kernel_choice(sn):
     """Kernel 0 (and no tuning at all): whole-lattice sweeps for replicas of
-    up to 2048 sites, tiles with the gather kernel above that (at 512 threads
-    for DipoleCutOff 1).  9^3 has no
-    tiling at rc = 3 (tile kernels refuse it) but a periodic colouring."""
+    up to 4096 sites that fit in LDS, tiles with the gather kernel above that
+    (at 512 threads for DipoleCutOff 1).  9^3 has no tiling at rc = 3 (tile
+    kernels refuse it) but a periodic colouring; 16^3 at DipoleCutOff 2 has
+    colours too large for the proposal slots of a whole-lattice sweep."""
     cases = [([sn.ReplicaParams(12, 12, 12, T=300)], True),
              ([sn.ReplicaParams(9, 9, 9, T=300)], True),
              ([sn.ReplicaParams(25, 25, 1, T=300)], True),
-             ([sn.ReplicaParams(16, 16, 16, T=300)], False),
-             ([sn.ReplicaParams(12, 12, 12, T=300), sn.ReplicaParams(16, 16, 16, T=300)], False)]
+             ([sn.ReplicaParams(16, 16, 16, T=300)], True),
+             ([sn.ReplicaParams(16, 16, 16, DipoleCutOff=2, T=300)], False),
+             ([sn.ReplicaParams(20, 20, 20, T=300)], False),
+             ([sn.ReplicaParams(12, 12, 12, T=300), sn.ReplicaParams(20, 20, 20, T=300)], False)]
     for reps, whole in cases:
         for tun in (None, sn.TuningParams()):
             with sn.StarryNight(reps, tun) as ctx:
                 cfg = ctx.sweep_config()
                 assert (cfg["tiles"] == 0) == whole, (reps[0].X, cfg)
-                if not whole:
+                if not whole and reps[-1].DipoleCutOff == 3:
                     assert cfg["lanes_per_site"] == 16
     # DipoleCutOff 1: 4 lanes a site (3 pairs); 5: 64 lanes a site (257
     # pairs, 5 a lane); kernel 1 reports the field kernel's 16 lanes
