@@ -437,8 +437,29 @@ def other_workloads(sn, log, autotune=True, only=None, schedule=None):
                             "one_call_per_replica: sn_gradient_device per replica; host_copy: sn_gradient, "
                             "the N x 3 field copied to host memory"}
 
+    def library_defaults():
+        # the library's own kernel choice (NULL tuning) on other cutoffs and
+        # sizes: 128^3 at DipoleCutOff 1, 2, 4, 5 (gather tiles), 64 replicas of
+        # 12^3 and 14^3 (whole-lattice sweeps), 2D 1024^2 at DipoleCutOff 3
+        res = {}
+        cases = [(f"128cube_rc{rc}", [sn.ReplicaParams(128, 128, 128, DipoleCutOff=rc, T=300, seed=rc)], 6)
+                 for rc in (1, 2, 4, 5)]
+        cases += [(f"64x{L}cube_rc3", [sn.ReplicaParams(L, L, L, T=300, seed=L + k) for k in range(64)], 40)
+                  for L in (12, 14)]
+        cases.append(("2d_1024sq_rc3", [sn.ReplicaParams(1024, 1024, 1, T=300, seed=1024)], 20))
+        for name, reps, sweeps in cases:
+            with sn.StarryNight(reps) as ctx:
+                for r in range(len(reps)):
+                    ctx.init_lattice_random(r)
+                cfg = ctx.sweep_config()
+                res[name] = {"site_updates_per_s": _timed_sweeps(ctx, sweeps, sum(r.N for r in reps)),
+                             "sweeps": sweeps, "whole_lattice": cfg["tiles"] == 0,
+                             "lanes_per_site": cfg["lanes_per_site"]}
+        return res
+
     table = {"2d_25x25_rc3": square_2d, "64cube_rc3_temperature_sweep": temperature_sweep,
-             "128cube_rc4_solid_solution_ramp": field_ramp, "32x48cube_adjoint": adjoint}
+             "128cube_rc4_solid_solution_ramp": field_ramp, "32x48cube_adjoint": adjoint,
+             "library_defaults": library_defaults}
     if only is not None and only not in table:
         raise SystemExit(f"unknown workload {only!r}; one of {sorted(table)}")
     out = {}
