@@ -624,11 +624,21 @@ int sn_create(const sn_params *params, int32_t n_replicas, const sn_tuning *tuni
     // 256-thread rate at 64^3 and 128^3).  DipoleCutOff 5 stays at 256
     // threads, 64 lanes x 5 pairs: 1.16x 512 threads of 64 lanes x 8 pairs,
     // 1.55x the field-update kernel at 128^3.
+    // It also runs 512 threads where its staged region leaves room for one
+    // 256-thread workgroup per CU (four waves) and 512 threads give a site
+    // twice the lanes: 128^3 at DipoleCutOff 4 (16^3 region), 32 lanes x 4
+    // pairs against 16 x 8, +4% (+17% half vacant).
+    int region = 0;
+    for (const TileShape &s : c->sched.shapes)
+        region = std::max(region, (s.tx + 2 * c->rc) * (s.ty + 2 * c->rc) *
+                                      gather_zstride(s.tz + (c->two_d ? 0 : 2 * c->rc)));
+    const bool one_per_cu = (size_t)(3 * region + 4 * c->prop_chunk) * sizeof(double) > 80 * 1024;
     const bool wide = max_class > kBlock / 4;
-    if (c->tuning.block_threads <= 0 && c->tuning.lanes_per_site <= 0 && (kern == 0 || kern == 2) && wide) {
+    if (c->tuning.block_threads <= 0 && c->tuning.lanes_per_site <= 0 && (kern == 0 || kern == 2) &&
+        (wide || one_per_cu)) {
         int G2, NPL2;
         lane_shape(512, G2, NPL2);
-        if (pick_sweep(G2, NPL2, false, 512)) { blk = 512; G = G2; NPL = NPL2; }
+        if (pick_sweep(G2, NPL2, false, 512) && (wide || G2 > G)) { blk = 512; G = G2; NPL = NPL2; }
     }
     c->blk = blk;
     c->G = G; c->NPL = NPL;
