@@ -424,6 +424,9 @@ static int setup_torus(sn_ctx *c)
     const bool greedy = gcost >= 0 && (lcost < 0 || gcost < lcost);
     c->torus_order = greedy ? go : lo;
     c->torus_cstart = greedy ? gs : ls;
+    // no empty colour before a replica's last site: the kernel's lookahead
+    // takes an empty colour for the end of the replica's sites
+    for (auto &st_ : c->torus_cstart) st_.erase(std::unique(st_.begin(), st_.end()), st_.end());
     c->torus_ncol = 0;
     for (const auto &st : c->torus_cstart) c->torus_ncol = std::max(c->torus_ncol, (int)st.size() - 1);
     std::vector<int> sites, off, cs;
