@@ -95,6 +95,13 @@ CASES = {
     # the library's choice above 4096 sites: DipoleCutOff 4 runs 512 threads
     # (one 256-thread workgroup's region per CU), 5 runs 64 lanes x 5 pairs
     "20cube_rc4_auto": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300, Efield=(0.0, 0.0, 0.04)), True, {}),
+    # 2D gather tiles at every cutoff: 4 x 1 (rc 1), 16 x 1 (rc 3), 16 x 2 (rc 4), ...
+    "2d_72x64_rc1_gather": (dict(X=72, Y=64, Z=1, DipoleCutOff=1, T=300, K=0.1), True, dict(kernel=2)),
+    "2d_72x64_rc2_gather": (dict(X=72, Y=64, Z=1, DipoleCutOff=2, T=300), False, dict(kernel=2)),
+    "2d_72x64_rc4_gather": (dict(X=72, Y=64, Z=1, DipoleCutOff=4, T=250, Efield=(0.01, 0.0, 0.0)), True,
+                            dict(kernel=2)),
+    "2d_80x80_rc5_gather": (dict(X=80, Y=80, Z=1, DipoleCutOff=5, T=300), True, dict(kernel=2)),
+    "2d_96x80_rc3_auto": (dict(X=96, Y=80, Z=1, T=300), True, {}),
     "20cube_rc5_auto": (dict(X=20, Y=20, Z=20, DipoleCutOff=5, T=300, K=0.1), True, {}),
     "18x18x18_rc3_auto": (dict(X=18, Y=18, Z=18, T=300), True, {}),
 }
