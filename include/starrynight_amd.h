@@ -269,6 +269,13 @@ int sn_stream(sn_ctx *ctx, void **hip_stream);
  * launch, the periodic colours). */
 int sn_sweep_config(sn_ctx *ctx, int32_t *tiles, int32_t *launches_per_sweep,
                     int32_t *lanes_per_site, int32_t *fine_colours);
+/* The sweep kernel the context runs (no counterpart in the reference, which
+ * has one code path): kernel 1 field update, 2 per-phase gather, 3 whole
+ * lattice; its threads per workgroup, lanes per site and pairs per lane (0
+ * for kernel 1, whose lanes split a site's stencil, not its pairs).  Any
+ * output may be NULL. */
+int sn_kernel_choice(sn_ctx *ctx, int32_t *kernel, int32_t *block_threads, int32_t *lanes_per_site,
+                     int32_t *pairs_per_lane);
 /* Time the tile-sweep kernel alone: runs nsweeps sweeps with a HIP event pair
  * around every tile-sweep launch and returns the mean launch duration (ms)
  * and the number of launches timed. */

@@ -26,7 +26,7 @@ EXPORTS = [
     "sn_mc_replay", "sn_counters", "sn_reset_counters", "sn_get_sweep_index", "sn_set_sweep_index",
     "sn_energy", "sn_gradient", "sn_moment",
     "sn_radial_order", "sn_dipole_potential", "sn_dipole_efield", "sn_recombination",
-    "sn_schedule_info", "sn_schedule_order", "sn_stream", "sn_sweep_config",
+    "sn_schedule_info", "sn_schedule_order", "sn_stream", "sn_sweep_config", "sn_kernel_choice",
     "sn_time_sweep_kernel", "sn_plan_schedule", "sn_autotune", "sn_reference_initial_lattice",
 ]
 
@@ -104,6 +104,7 @@ def lib():
         L.sn_schedule_order.argtypes = [vp, C.c_int32, ip, ip]
         L.sn_stream.argtypes = [vp, P(vp)]
         L.sn_sweep_config.argtypes = [vp, ip, ip, ip, ip]
+        L.sn_kernel_choice.argtypes = [vp, ip, ip, ip, ip]
         L.sn_time_sweep_kernel.argtypes = [vp, C.c_int32, dp, ip]
         L.sn_autotune.argtypes = [P(Params), C.c_int32, C.c_int32, dp, dp, C.c_int32, C.c_int32, P(Tuning), dp]
         L.sn_reference_initial_lattice.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_char_p, C.c_uint32,

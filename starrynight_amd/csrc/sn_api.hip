@@ -1684,6 +1684,18 @@ int sn_stream(sn_ctx *c, void **s)
     return SN_OK;
 }
 
+int sn_kernel_choice(sn_ctx *c, int32_t *kernel, int32_t *block_threads, int32_t *lanes, int32_t *pairs)
+{
+    if (!c) return SN_ERR_INVALID;
+    if (!c->can_sweep) return fail(c, SN_ERR_UNSUPPORTED, "sweeps unavailable: " + c->sweep_err);
+    const int k = c->torus ? 3 : c->use_field ? 1 : 2;
+    if (kernel) *kernel = k;
+    if (block_threads) *block_threads = k == 3 ? c->torus_blk : k == 1 ? kBlock : c->blk;
+    if (lanes) *lanes = k == 3 ? c->torus_G : k == 1 ? kFieldG : c->G;
+    if (pairs) *pairs = k == 3 ? c->torus_NPL : k == 1 ? 0 : c->NPL;
+    return SN_OK;
+}
+
 int sn_sweep_config(sn_ctx *c, int32_t *tiles, int32_t *launches, int32_t *lanes, int32_t *fine)
 {
     if (!c) return SN_ERR_INVALID;

@@ -316,6 +316,15 @@ class StarryNight:
         return {"tiles": t.value, "launches_per_sweep": l.value, "lanes_per_site": g.value,
                 "fine_colours": f.value}
 
+    def kernel_choice(self):
+        """The sweep kernel this context runs (1 field update, 2 gather, 3
+        whole lattice), its threads per workgroup, lanes per site and pairs
+        per lane."""
+        k, b, g, n = C.c_int32(0), C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        self._chk(lib().sn_kernel_choice(self._h, C.byref(k), C.byref(b), C.byref(g), C.byref(n)))
+        return {"kernel": k.value, "block_threads": b.value, "lanes_per_site": g.value,
+                "pairs_per_lane": n.value}
+
     def time_sweep_kernel(self, nsweeps: int):
         ms, n = C.c_double(0), C.c_int32(0)
         self._chk(lib().sn_time_sweep_kernel(self._h, int(nsweeps), C.byref(ms), C.byref(n)))

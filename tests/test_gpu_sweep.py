@@ -290,6 +290,22 @@ def test_default_kernel_choice(sn):
                 assert (cfg["tiles"] == 0) == whole, (reps[0].X, cfg)
                 if not whole and reps[-1].DipoleCutOff == 3:
                     assert cfg["lanes_per_site"] == 16
+    # the whole choice (kernel, threads, lanes x pairs) for NULL tuning
+    R = sn.ReplicaParams
+    for reps, want in (([R(128, 128, 128, T=300)], (2, 256, 16, 4)),
+                       ([R(128, 128, 128, DipoleCutOff=4, T=300)], (2, 512, 32, 4)),
+                       ([R(32, 32, 32, DipoleCutOff=5, T=300)], (2, 256, 64, 5)),
+                       ([R(32, 32, 32, DipoleCutOff=1, T=300)], (2, 512, 4, 1)),
+                       ([R(64, 64, 64, DipoleCutOff=2, T=300)], (2, 256, 4, 4)),
+                       ([R(256, 256, 1, T=300)], (2, 256, 16, 1)),
+                       ([R(12, 12, 12, T=300)], (3, 256, 16, 4)),
+                       ([R(25, 25, 1, T=300)], (3, 512, 8, 2))):
+        with sn.StarryNight(reps) as ctx:
+            kc = ctx.kernel_choice()
+            got = (kc["kernel"], kc["block_threads"], kc["lanes_per_site"], kc["pairs_per_lane"])
+            assert got == want, (reps[0], got)
+    with sn.StarryNight(R(32, 32, 32, T=300), sn.TuningParams(kernel=1)) as ctx:
+        assert ctx.kernel_choice() == {"kernel": 1, "block_threads": 256, "lanes_per_site": 16, "pairs_per_lane": 0}
     # DipoleCutOff 1: 4 lanes a site (3 pairs); 5: 64 lanes a site (257
     # pairs, 5 a lane); kernel 1 reports the field kernel's 16 lanes
     for rc, tun, lanes in ((1, None, 4), (1, sn.TuningParams(kernel=2, block_threads=256), 4),
