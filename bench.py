@@ -239,7 +239,8 @@ def main():
     ctx = sn.StarryNight(rp, tun)
     ctx.init_lattice_random(0)
     cfg = ctx.sweep_config()
-    log(f"[bench] {L}^3 rc={rc} N={N} neighbours={n_nb} F={F} sweep config {cfg}")
+    kc = ctx.kernel_choice()
+    log(f"[bench] {L}^3 rc={rc} N={N} neighbours={n_nb} F={F} sweep config {cfg}, kernel {kc}")
     ctx.MC_sweeps(args.warmup)
     ctx.reset_counters()
     # one GPU, one process: the barrier is trivial; the library stream and
@@ -310,6 +311,7 @@ def main():
                    "X": L, "Y": L, "Z": L, "DipoleCutOff": rc, "CageStrain": 1.0, "Efield": [0, 0, 0],
                    "T": 300, "neighbours": n_nb, "tiles": cfg["tiles"],
                    "launches_per_sweep": cfg["launches_per_sweep"], "lanes_per_site": cfg["lanes_per_site"],
+                   "block_threads": kc["block_threads"], "pairs_per_lane": kc["pairs_per_lane"],
                    "fine_colours": cfg["fine_colours"],
                    "schedule": ("autotuned " + str(tun.__dict__)) if tun is not None else "built-in default"},
         "roofline": roof,
@@ -451,10 +453,8 @@ def other_workloads(sn, log, autotune=True, only=None, schedule=None):
             with sn.StarryNight(reps) as ctx:
                 for r in range(len(reps)):
                     ctx.init_lattice_random(r)
-                cfg = ctx.sweep_config()
                 res[name] = {"site_updates_per_s": _timed_sweeps(ctx, sweeps, sum(r.N for r in reps)),
-                             "sweeps": sweeps, "whole_lattice": cfg["tiles"] == 0,
-                             "lanes_per_site": cfg["lanes_per_site"]}
+                             "sweeps": sweeps, **ctx.kernel_choice()}
         return res
 
     table = {"2d_25x25_rc3": square_2d, "64cube_rc3_temperature_sweep": temperature_sweep,
