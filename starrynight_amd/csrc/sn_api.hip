@@ -453,6 +453,9 @@ static int setup_torus(sn_ctx *c)
         while (NPL * G < np) NPL <<= 1;
         blk = c->torus_mc * G > 512 ? 1024 : c->torus_mc * G > 256 ? 512 : 256;
         if (!pick_torus(G, NPL, false, blk) && blk < 1024 && pick_torus(G, NPL, false, 1024)) blk = 1024;
+        if (!pick_torus(G, NPL, false, blk))   // fewer threads: several rounds per colour
+            for (int b : {512, 256})
+                if (pick_torus(G, NPL, false, b)) { blk = b; break; }
     }
     c->torus_G = G; c->torus_NPL = NPL; c->torus_blk = blk;
     // proposal slots: whole runs of colours per draw, as many as LDS leaves room

@@ -92,6 +92,9 @@ CASES = {
     "10cube_rc1_torus": (dict(X=10, Y=10, Z=10, DipoleCutOff=1, T=300, K=0.1), True, dict(kernel=3)),
     "24x20x1_rc5_torus": (dict(X=24, Y=20, Z=1, DipoleCutOff=5, T=300), True, dict(kernel=3)),
     "12cube_rc1_auto": (dict(X=12, Y=12, Z=12, DipoleCutOff=1, T=300), True, {}),
+    # requested lanes whose one-round thread count has no instance run several rounds
+    "12cube_rc2_torus_lanes4": (dict(X=12, Y=12, Z=12, DipoleCutOff=2, T=300), True,
+                                dict(kernel=3, lanes_per_site=4)),
     # the library's choice above 4096 sites: DipoleCutOff 4 runs 512 threads
     # (one 256-thread workgroup's region per CU), 5 runs 64 lanes x 5 pairs
     "20cube_rc4_auto": (dict(X=20, Y=20, Z=20, DipoleCutOff=4, T=300, Efield=(0.0, 0.0, 0.04)), True, {}),
