@@ -300,7 +300,7 @@ def test_default_kernel_choice(sn):
                        ([R(32, 32, 32, DipoleCutOff=5, T=300)], (2, 256, 64, 5)),
                        ([R(32, 32, 32, DipoleCutOff=1, T=300)], (2, 512, 4, 1)),
                        ([R(64, 64, 64, DipoleCutOff=2, T=300)], (2, 256, 4, 4)),
-                       ([R(256, 256, 1, T=300)], (2, 256, 16, 1)),
+                       ([R(256, 256, 1, T=300)], (2, 256, 8, 2)),
                        ([R(12, 12, 12, T=300)], (3, 256, 16, 4)),
                        ([R(25, 25, 1, T=300)], (3, 512, 8, 2))):
         with sn.StarryNight(reps) as ctx:
